@@ -1,0 +1,274 @@
+"""Python binding of the MI355X HEOM library (libheom_hip.so, C ABI in include/heom_hip.h).
+
+The binding is a thin ctypes layer: every call goes to the HIP library, which fails
+loudly (HeomError) when it is missing or when no GPU is present -- there is no CPU
+fallback.  ``Solver`` mirrors the reference's solver interface for this path
+(population_dynamics_solver: reset / step_forward / hierarchy_top / hierarchy_norm /
+hierarchy_mmult_left, dm-heom/include/heom/population_dynamics_solver.hpp and
+dm-heom/include/heom/solver_base.hpp) on top of one library handle.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(ROOT, "lib", "libheom_hip.so")
+HEADER = os.path.join(os.path.dirname(ROOT), "include", "heom_hip.h")
+
+HEOM_OK = 0
+HEOM_E_INVALID = -1
+HEOM_E_UNSUPPORTED = -2
+HEOM_E_DEVICE = -3
+HEOM_E_MEMORY = -4
+HEOM_PRECISION_FP64 = 0
+
+# every function include/heom_hip.h declares
+EXPORTED = [
+    "heom_create", "heom_destroy", "heom_last_error", "heom_abi_version",
+    "heom_set_hamiltonian", "heom_set_mask", "heom_get_mask", "heom_set_state", "heom_set_top",
+    "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
+    "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
+    "heom_rk4_step_bytes",
+]
+
+
+class HeomError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"heom error {code}: {msg}")
+        self.code = code
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("states", ctypes.c_int32), ("matsubaras", ctypes.c_int32), ("baths", ctypes.c_int32),
+        ("baths_max_per_state", ctypes.c_int32), ("width", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("matrices", ctypes.c_int64),
+        ("coupling", ctypes.c_void_p), ("hamiltonian", ctypes.c_void_p),
+        ("ado_tuples", ctypes.c_void_p), ("ado_plus", ctypes.c_void_p), ("ado_minus", ctypes.c_void_p),
+        ("pf_same_sum", ctypes.c_void_p), ("pf_same", ctypes.c_void_p), ("pf_circ", ctypes.c_void_p),
+        ("cbk", ctypes.c_void_p),
+    ]
+
+
+class TimingInfo(ctypes.Structure):
+    _fields_ = [("rk4_steps", ctypes.c_int64), ("rk4_seconds", ctypes.c_double),
+                ("rhs_launches", ctypes.c_int64), ("last_call_seconds", ctypes.c_double)]
+
+
+_lib = None
+
+
+def build() -> str:
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", ROOT], check=True)
+    return LIB_PATH
+
+
+def load():
+    """Load libheom_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HeomError(HEOM_E_UNSUPPORTED, f"libheom_hip.so not built: {LIB_PATH}")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double
+        sig = {
+            "heom_create": [ctypes.POINTER(Desc), ctypes.POINTER(vp)],
+            "heom_destroy": [vp],
+            "heom_abi_version": [],
+            "heom_set_hamiltonian": [vp, vp],
+            "heom_set_mask": [vp, vp],
+            "heom_get_mask": [vp, vp],
+            "heom_set_state": [vp, vp],
+            "heom_set_top": [vp, vp],
+            "heom_get_state": [vp, vp],
+            "heom_get_top": [vp, vp],
+            "heom_hierarchy_norm": [vp, dbl, vp],
+            "heom_rhs": [vp, vp],
+            "heom_step_rk4": [vp, dbl, i64],
+            "heom_propagate": [vp, dbl, i64, i64, vp],
+            "heom_mmult_left": [vp, vp],
+            "heom_sync": [vp],
+            "heom_timing": [vp, ctypes.POINTER(TimingInfo)],
+            "heom_device_bytes": [vp, ctypes.POINTER(i64)],
+            "heom_rk4_step_bytes": [vp, ctypes.POINTER(i64)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.heom_last_error.argtypes = []
+        L.heom_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != HEOM_OK:
+        raise HeomError(rc, load().heom_last_error().decode())
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128_flat(m) -> np.ndarray:
+    m = np.asarray(m, dtype=np.complex128)
+    return _f64(np.stack([m.real, m.imag], axis=-1).reshape(-1))
+
+
+class Handle:
+    """One hierarchy on the GPU (owns all its device memory)."""
+
+    def __init__(self, *, states, matsubaras, baths, coupling, hamiltonian, tuples, plus, minus,
+                 pf_same_sum, pf_same, pf_circ, cbk, precision=HEOM_PRECISION_FP64):
+        L = load()
+        coupling = np.ascontiguousarray(coupling, dtype=np.int32)
+        keep = dict(
+            coupling=coupling,
+            hamiltonian=_f64(hamiltonian).reshape(-1),
+            tuples=np.ascontiguousarray(tuples, dtype=np.int32).reshape(-1),
+            plus=np.ascontiguousarray(plus, dtype=np.int32).reshape(-1),
+            minus=np.ascontiguousarray(minus, dtype=np.int32).reshape(-1),
+            pf_same_sum=_f64(pf_same_sum), pf_same=_f64(pf_same), pf_circ=_f64(pf_circ), cbk=_f64(cbk),
+        )
+        width = int(baths) * int(matsubaras)
+        d = Desc()
+        d.states, d.matsubaras, d.baths = int(states), int(matsubaras), int(baths)
+        d.baths_max_per_state = int(coupling.shape[1]) if coupling.ndim == 2 else int(coupling.size // states)
+        d.width = width
+        d.precision = int(precision)
+        d.matrices = int(keep["tuples"].size // max(width, 1))
+        d.coupling = keep["coupling"].ctypes.data
+        d.hamiltonian = keep["hamiltonian"].ctypes.data
+        d.ado_tuples = keep["tuples"].ctypes.data
+        d.ado_plus = keep["plus"].ctypes.data
+        d.ado_minus = keep["minus"].ctypes.data
+        d.pf_same_sum = keep["pf_same_sum"].ctypes.data
+        d.pf_same = keep["pf_same"].ctypes.data
+        d.pf_circ = keep["pf_circ"].ctypes.data
+        d.cbk = keep["cbk"].ctypes.data
+        h = ctypes.c_void_p()
+        _check(L.heom_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.states = d.states
+        self.matrices = d.matrices
+        self.width = width
+
+    @classmethod
+    def from_instance(cls, inst, precision=HEOM_PRECISION_FP64) -> "Handle":
+        """Create from any object carrying the reference instance arrays (states,
+        matsubaras, baths, coupling, hamiltonian, tuples, plus, minus, pf_*, cbk)."""
+        return cls(states=inst.states, matsubaras=inst.matsubaras, baths=inst.baths,
+                   coupling=np.asarray(inst.coupling).reshape(inst.states, -1),
+                   hamiltonian=inst.hamiltonian, tuples=inst.tuples, plus=inst.plus, minus=inst.minus,
+                   pf_same_sum=inst.pf_same_sum, pf_same=inst.pf_same, pf_circ=inst.pf_circ,
+                   cbk=inst.cbk, precision=precision)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(load().heom_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # sizes
+    @property
+    def state_size(self) -> int:
+        return self.matrices * self.states * self.states * 2
+
+    # inputs
+    def set_state(self, state) -> None:
+        s = _f64(state)
+        if s.size != self.state_size:
+            raise ValueError("state has the wrong size")
+        _check(load().heom_set_state(self._h, s.ctypes.data))
+
+    def set_top(self, top) -> None:
+        t = _c128_flat(top)
+        _check(load().heom_set_top(self._h, t.ctypes.data))
+
+    def set_hamiltonian(self, H) -> None:
+        h = _c128_flat(H) if np.iscomplexobj(H) else _f64(H)
+        _check(load().heom_set_hamiltonian(self._h, h.ctypes.data))
+
+    def set_mask(self, mask) -> None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.size != self.matrices:
+            raise ValueError("mask has the wrong size")
+        _check(load().heom_set_mask(self._h, m.ctypes.data))
+
+    def get_mask(self) -> np.ndarray:
+        m = np.zeros(self.matrices, dtype=np.uint8)
+        _check(load().heom_get_mask(self._h, m.ctypes.data))
+        return m
+
+    # outputs
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.state_size)
+        _check(load().heom_get_state(self._h, out.ctypes.data))
+        return out
+
+    def get_top(self) -> np.ndarray:
+        out = np.empty(self.states * self.states * 2)
+        _check(load().heom_get_top(self._h, out.ctypes.data))
+        t = out.reshape(self.states, self.states, 2)
+        return t[..., 0] + 1j * t[..., 1]
+
+    def norms(self, threshold: float = -1.0) -> np.ndarray:
+        out = np.empty(self.matrices)
+        _check(load().heom_hierarchy_norm(self._h, float(threshold), out.ctypes.data))
+        return out
+
+    # evaluation
+    def rhs(self) -> np.ndarray:
+        out = np.empty(self.state_size)
+        _check(load().heom_rhs(self._h, out.ctypes.data))
+        return out
+
+    def step_rk4(self, h: float, nsteps: int) -> None:
+        _check(load().heom_step_rk4(self._h, float(h), int(nsteps)))
+
+    def propagate(self, h: float, nsteps: int, observe_steps: int) -> np.ndarray:
+        rows = nsteps // observe_steps + 1
+        out = np.empty(rows * self.states * self.states * 2)
+        _check(load().heom_propagate(self._h, float(h), int(nsteps), int(observe_steps), out.ctypes.data))
+        t = out.reshape(rows, self.states, self.states, 2)
+        return t[..., 0] + 1j * t[..., 1]
+
+    def mmult_left(self, mx) -> None:
+        m = _c128_flat(mx)
+        _check(load().heom_mmult_left(self._h, m.ctypes.data))
+
+    def sync(self) -> None:
+        _check(load().heom_sync(self._h))
+
+    def timing(self) -> TimingInfo:
+        t = TimingInfo()
+        _check(load().heom_timing(self._h, ctypes.byref(t)))
+        return t
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_int64()
+        _check(load().heom_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def rk4_step_bytes(self) -> int:
+        b = ctypes.c_int64()
+        _check(load().heom_rk4_step_bytes(self._h, ctypes.byref(b)))
+        return b.value

@@ -1,0 +1,158 @@
+/*
+ * heom_hip.h -- C ABI of the MI355X HEOM solver library (libheom_hip.so).
+ *
+ * This is the drop-in boundary for DM-HEOM's ODE plug-in point: the reference
+ * instantiates population_dynamics_solver<heom::ode, num::meta_stepper,
+ * heom::hierarchy_norm> (dm-heom/include/heom/population_dynamics_solver.hpp:31-32,
+ * dm-heom/src/app_population_dynamics.cpp:59-60); everything that template drives on
+ * the device -- the heom_ode RHS, the RK4 stepper with rk_weighted_add, the
+ * hierarchy_norm kernel, mmult_left, the state double buffer and its read/write-back
+ * -- sits behind the functions below.  Plain C types only; one handle owns all device
+ * memory of one hierarchy; no pointer passed in is retained after a call returns.
+ *
+ * Data contract at the boundary is the reference's (dm-heom/include/heom/instance.hpp
+ * :119-171): ADOs in the graph's uid order, each N x N complex matrix row-major with
+ * interleaved (re, im) doubles; tables int32 [M * W], invalid edge = any negative value;
+ * prefactors as produced by instance::precompute_baths_input (SI units).  Inside, the
+ * library stores ADOs structure-of-arrays by matrix element in its own ADO order; the
+ * layout never leaks through the ABI.
+ *
+ * Errors: every function returns HEOM_OK (0) or a negative HEOM_E_* code and sets a
+ * thread-local message readable with heom_last_error().  No C++ exception crosses the
+ * ABI (the reference's ocl::error_handler / config_error throw instead,
+ * dm-heom/src/heom/ode.cpp:52, dm-heom/include/heom/config_error.hpp:17-23).
+ *
+ * Threading: one handle is used by one host thread at a time; calls on a handle are
+ * ordered on its own HIP stream.  Calls that return data to the host synchronise.
+ */
+#ifndef HEOM_HIP_H
+#define HEOM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEOM_ABI_VERSION 1
+
+enum heom_status {
+    HEOM_OK = 0,
+    HEOM_E_INVALID = -1,     /* bad argument or descriptor */
+    HEOM_E_UNSUPPORTED = -2, /* valid but not supported by this build (e.g. N too large) */
+    HEOM_E_DEVICE = -3,      /* HIP runtime error */
+    HEOM_E_MEMORY = -4       /* host or device allocation failed */
+};
+
+enum heom_precision {
+    HEOM_PRECISION_FP64 = 0 /* complex128 ADO storage, FP64 RHS and RK4 */
+};
+
+/* Static description of one hierarchy (the reference's heom::instance plus the
+ * generated #defines of solver_base::ocl_source_header,
+ * dm-heom/include/heom/solver_base.hpp:179-195). */
+typedef struct heom_desc {
+    int32_t states;              /* N = NUM_STATES */
+    int32_t matsubaras;          /* K = NUM_MATSUBARAS (modes per bath) */
+    int32_t baths;               /* B */
+    int32_t baths_max_per_state; /* BATHS_MAX_PER_STATE */
+    int32_t width;               /* W = ADO_WIDTH = B * K */
+    int32_t precision;           /* enum heom_precision */
+    int64_t matrices;            /* M = NUM_MATRICES */
+    const int32_t* coupling;     /* BATHS_COUPLING_LIST [N * baths_max_per_state], <0 = none */
+    const double* hamiltonian;   /* [N * N * 2] */
+    const int32_t* ado_tuples;   /* [M * W] */
+    const int32_t* ado_plus;     /* [M * W] */
+    const int32_t* ado_minus;    /* [M * W] */
+    const double* pf_same_sum;   /* [2 * M] */
+    const double* pf_same;       /* [2 * B] */
+    const double* pf_circ;       /* [2 * B] */
+    const double* cbk;           /* [2 * B * K] */
+} heom_desc;
+
+typedef struct heom_handle heom_handle;
+
+/* Timing of the calls made so far on a handle (device time from HIP events). */
+typedef struct heom_timing_info {
+    int64_t rk4_steps;         /* RK4 steps taken */
+    double rk4_seconds;        /* device time spent in RK4 stepping */
+    int64_t rhs_launches;      /* stage kernel launches (4 per RK4 step, 1 per heom_rhs) */
+    double last_call_seconds;  /* device time of the last stepping call */
+} heom_timing_info;
+
+/* ---- lifetime ----------------------------------------------------------------- */
+
+/* Replaces heom::ode::ode + ode::initialise (dm-heom/src/heom/ode.cpp:35-95) and the
+ * state buffers of population_dynamics_solver::initialize
+ * (dm-heom/include/heom/population_dynamics_solver.hpp:122-159).  Copies every array
+ * of desc to the device.  The initial state is all zeros and the mask all TRUE. */
+int heom_create(const heom_desc* desc, heom_handle** out);
+int heom_destroy(heom_handle* handle);
+/* message of the last failing call on this thread ("" if none) */
+const char* heom_last_error(void);
+int heom_abi_version(void);
+
+/* ---- inputs -------------------------------------------------------------------- */
+
+/* ode::update_hamiltonian (dm-heom/src/heom/ode.cpp:197-203) */
+int heom_set_hamiltonian(heom_handle* handle, const double* hamiltonian /* [N*N*2] */);
+/* solver_base::update_hierarchy_mask (dm-heom/include/heom/solver_base.hpp:292-299):
+ * values 0 = FALSE, 1 = TRUE, 2 = PROTECTED; as in heom_ode.cl:199,237,297,360 a term
+ * is dropped when the NEIGHBOUR it reads is masked. */
+int heom_set_mask(heom_handle* handle, const uint8_t* mask /* [M] */);
+int heom_get_mask(heom_handle* handle, uint8_t* mask /* [M] */);
+/* population_dynamics_solver::reset (population_dynamics_solver.hpp:161-175):
+ * the whole state, reference layout. */
+int heom_set_state(heom_handle* handle, const double* state /* [M*N*N*2] */);
+/* solver_base::update_hierarchy_top (solver_base.hpp:283-289): overwrite ADO 0 only */
+int heom_set_top(heom_handle* handle, const double* top /* [N*N*2] */);
+
+/* ---- outputs ------------------------------------------------------------------- */
+
+/* solver_base::hierarchy (solver_base.hpp:213-216) */
+int heom_get_state(heom_handle* handle, double* state /* [M*N*N*2] */);
+/* solver_base::hierarchy_top (solver_base.hpp:219-222) */
+int heom_get_top(heom_handle* handle, double* top /* [N*N*2] */);
+/* solver_base::update_hierarchy_norm + hierarchy_norm (solver_base.hpp:224-228,
+ * 270-273; kernel cl/heom/hierarchy_norm.cl:15-45): norms[n] = sum_ij |sigma_n,ij|^2 in
+ * uid order; ADOs with norm - threshold <= 0 that are not PROTECTED are masked
+ * (a negative threshold never triggers). */
+int heom_hierarchy_norm(heom_handle* handle, double threshold, double* norms /* [M] */);
+
+/* ---- evaluation and stepping ----------------------------------------------------- */
+
+/* One RHS evaluation f(current state) into out (reference layout): ode::solve's pure
+ * evaluation (dm-heom/src/heom/ode.cpp:97-117, heom_ode.cl:430-434), as used by the
+ * rate-kernel app (dm-heom/src/app_rate_kernel.cpp:91-93). The state is unchanged. */
+int heom_rhs(heom_handle* handle, double* out /* [M*N*N*2] */);
+/* nsteps classical RK4 steps of size h (seconds): population_dynamics_solver::
+ * step_forward (population_dynamics_solver.hpp:233-265) with the stepper's heom_ode and
+ * rk_weighted_add launches fused into one kernel per stage.  Asynchronous. */
+int heom_step_rk4(heom_handle* handle, double h, int64_t nsteps);
+/* nsteps RK4 steps, recording the top ADO on the device every observe_steps steps
+ * (first record = the state before stepping); rows = nsteps / observe_steps + 1 records
+ * of N*N*2 doubles are written to tops at the end.  The reference reads the top back
+ * synchronously after every step_forward(observe_steps) (app_population_dynamics.cpp:
+ * 92-106); this does the same without a host round trip per observation. */
+int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
+                   double* tops /* [(nsteps/observe_steps + 1) * N*N*2] */);
+/* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
+ * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */
+int heom_mmult_left(heom_handle* handle, const double* mx /* [N*N*2] */);
+/* wait for all work queued on the handle */
+int heom_sync(heom_handle* handle);
+int heom_timing(heom_handle* handle, heom_timing_info* info);
+
+/* ---- introspection ------------------------------------------------------------- */
+
+/* bytes of device memory held by the handle */
+int heom_device_bytes(heom_handle* handle, int64_t* bytes);
+/* algorithmic HBM bytes of one RK4 step of the whole hierarchy
+ * (16 state passes x M x N^2 x sizeof(complex)) */
+int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEOM_HIP_H */

@@ -1,0 +1,806 @@
+/*
+ * heom_oracle.c -- CPU oracle (FP64) for DM-HEOM's population-dynamics and
+ * linear-absorption path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker.  Only tests/, the smoke check
+ * and the benchmark's CPU-baseline leg load it; the product never links or calls it.
+ *
+ * Each function restates the reference's algorithm statement by statement, in FP64,
+ * compiled with -ffp-contract=off so that every statement rounds as written.  Paths
+ * are relative to the reference repository root (dm-heom/...).
+ *
+ * Parity status of each part (how it is pinned is in the tests):
+ *   partitions / graph   pinned: closed-form counts, brute-force enumeration, a step-by-step
+ *                        restatement of the reference generator, plus/minus invariants
+ *   precompute           pinned: 50-digit mpmath evaluation of the formulas; Pade table
+ *                        checked against the Bose function it approximates
+ *   rhs                  pinned: dense-Liouvillian brute force, no-bath von Neumann limit
+ *                        (expm), trace conservation, Hermiticity
+ *   rk4                  pinned: order-4 convergence against expm of the dense Liouvillian;
+ *                        the exact call sequence of the un-vendored noma/num stepper is
+ *                        parity unpinned (classical tableau assumed)
+ *   norms, mmult_left,   pinned: textbook definitions checked in numpy
+ *   trace
+ */
+#include "heom_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ħ as used by the kernels (cl/heom/common.cl:45) and the host
+ * (include/heom/constants.hpp:22); k_B (include/heom/constants.hpp:23). */
+static const double ORACLE_HBAR = 1.05457148e-34;
+static const double ORACLE_KB = 1.380650239684840458953781387048428305608616777679979316e-23;
+
+/* Bosonic Pade expansion tables, rows of 10, row K-1 used for K "matsubaras"
+ * (src/heom/constants.cpp:13-76).  These are data: reproduced value for value. */
+#define ORACLE_MAX_PADE 10
+static const long double oracle_eta_pade[100] = {
+    2.5L, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+    5.967175818975844784692668313888340351402L, 1.032824181024155215307331686111659648598L,
+    0, 0, 0, 0, 0, 0, 0, 0,
+    11.19885869716451815284926959121234309769L, 1.300914314466752848423178399066599315543L,
+    1.000226988368728998727552009721057586765L, 0, 0, 0, 0, 0, 0, 0,
+    18.0790807744036381353279225690526377613L, 1.905605223763320862837482220696034638651L,
+    1.015313588078062122619093914564704203251L, 1.000000413754978879215501295686623396803L,
+    0, 0, 0, 0, 0, 0,
+    26.58655772186703141593168716080327785988L, 2.800146619916205478116315260146686752654L,
+    1.11303340660549395961527574088616034253L, 1.000262251326821823459883045067529050956L,
+    1.000000000284447322876838793096345993981L, 0, 0, 0, 0, 0,
+    36.71706591275454016953866376131015396352L, 3.915524608369778138857104011713377210806L,
+    1.36060762304475364762294649920250800437L, 1.006800099494035677225303524345587578974L,
+    1.000001756336801785832037797152047562438L, 1.000000000000090580923944406276325679894L,
+    0, 0, 0, 0,
+    48.46945901876815939911017902590799133678L, 5.221358700226750582830885303574125272024L,
+    1.76101413086885578701660606207054290627L, 1.048001360645944284478966960182497444605L,
+    1.000166783806547433957971903891016917251L, 1.000000005683742497388628469835034317229L,
+    1.000000000000000015216762274538791805838L, 0, 0, 0,
+    61.84333899016684164883279780411421181762L, 6.709368642028293738757159220036413607145L,
+    2.276634739389544433995764512332145784496L, 1.167662545543253700136635145976136601895L,
+    1.002993112803799185748365319635086694072L, 1.000001970058157018499040773410845478966L,
+    1.000000000010110274028760159309306116011L, 1.000000000000000000001477065185853899797L,
+    0, 0,
+    76.83854093567190329220602502323513693626L, 8.377617889976198321037045566932019912232L,
+    2.875743265352905559741498484957445572073L, 1.386916450076667643674849716161379189731L,
+    1.021092605957815260732482889860564891757L, 1.000088840018504469980284336164406834212L,
+    1.000000012945994679324047582718370173438L, 1.000000000000010773303766311318241171588L,
+    1.000000000000000000000000088652435318708L, 0,
+    93.45498712043917746792411329602543487242L, 10.22570574466955029693172438665880309108L,
+    3.544527893692194584758656451673749774362L, 1.692413746219203914341865930002582507947L,
+    1.081051704862394372162728930732659549555L, 1.00131233986343205708864037801373384889L,
+    1.000001450202211357298203645556646479551L, 1.000000000051835942172273790638144172787L,
+    1.000000000000000007321793190694778206164L, 1.000000000000000000000000000003467497243L};
+
+static const long double oracle_xi_pade[100] = {
+    7.745966692414833770358530799564799221666L, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+    19.49961875292266554276226912855528560143L, 6.305939144224808474630860431551181929313L,
+    0, 0, 0, 0, 0, 0, 0, 0,
+    36.11928942365857887916363930568047443027L, 12.95828669566379113836433196980720441554L,
+    6.283290335721270010976001332769855412497L, 0, 0, 0, 0, 0, 0, 0,
+    57.78794000633463555484777318092706162218L, 20.56259756752951982974170033395653675754L,
+    12.57995038432843325319503152817888327099L, 6.283185452295109512774977729104135871222L,
+    0, 0, 0, 0, 0, 0,
+    84.53692559026397942121263237089439381387L, 29.57927623718706979609581199085607351807L,
+    19.00468956251068382661480700055507075954L, 12.56654232705146493528755058935472519371L,
+    6.28318530726039477349210954220860566044L, 0, 0, 0, 0, 0,
+    116.3739347608330350073189696097089226231L, 40.21293928856715127416984659035932828755L,
+    25.82783232953028238453829864275777795836L, 18.8562400267953494905521831656114592347L,
+    12.56637153917480629098401692212665860923L, 6.28318530717960815500107855468522104325L,
+    0, 0, 0, 0,
+    153.3015628393653726331899655194706876159L, 52.52412157944826796685914142534634337648L,
+    33.34898703451500336730287935111300571215L, 25.19900528903505310331751346668259596763L,
+    18.84968412056844144448843441419311682201L, 12.56637061688110415733890859371956448695L,
+    6.283185307179586480075153567585578254575L, 0, 0, 0,
+    195.3208905359343070977627172198676228279L, 66.52807871550278259674646939491255055216L,
+    41.76755706259573674247409323452395574559L, 31.72895561746715247842724612158554657776L,
+    25.13585329999435832854980649807171916496L, 18.84955718260564459859203580283272390792L,
+    12.56637061436306356083304954158209077981L, 6.283185307179586476925556323382874292731L,
+    0, 0,
+    242.43243633332578223865362314113358865L, 82.22872388545609111729841286466997903147L,
+    51.17127400682872814625269028495752604681L, 38.64346654893615544641179566060655896253L,
+    31.44483562147124840697966631064127684252L, 25.13281634168110515436812140082770700199L,
+    18.84955592869136612218434285960431278578L, 12.56637061435917662145272429955102313797L,
+    6.283185307179586476925286781031145512608L, 0,
+    294.6364754696112018848669150774790893224L, 99.62709625387022543419284510745749131322L,
+    61.58845079917599582892471747985579618666L, 46.1203736291541763005458261878858666512L,
+    37.84390364517245088090747934823213310617L, 31.41734004510526384373874233227272273452L,
+    25.13274227487208277473085335076650873783L, 18.84955592156405141516275123243075673053L,
+    12.56637061435917295608825501073938367082L, 6.283185307179586476925286766559517951084L};
+
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ======================================================================== */
+/* integer partitions (src/heom/integer_partition.cpp)                       */
+/* ======================================================================== */
+
+/* partitions of n into exactly k parts: p(n,k) = p(n-k,k) + p(n-1,k-1)
+ * (integer_partition.cpp:20-26) */
+static int64_t exact_parts(int n, int k)
+{
+    if (n == 0 && k == 0)
+        return 1;
+    if (n <= 0 || k <= 0)
+        return 0;
+    return exact_parts(n - k, k) + exact_parts(n - 1, k - 1);
+}
+
+/* integer_partition.cpp:30-38 */
+int64_t oracle_partition_count(int n, int s)
+{
+    int64_t total = 0;
+    for (int p = 1; p <= s; ++p)
+        total += exact_parts(n, p);
+    return total;
+}
+
+/* Copy the current partition x[1..len] into row r of out (s columns, zero padded). */
+static void emit_partition(const int32_t* x, int len, int s, int32_t* out, int64_t* r)
+{
+    int32_t* row = out + (*r) * s;
+    int p = 0;
+    for (; p < len; ++p)
+        row[p] = x[p + 1];
+    for (; p < s; ++p)
+        row[p] = 0;
+    ++(*r);
+}
+
+/* The reference enumerates the partitions of n starting from 1+1+...+1 and moving to
+ * the next partition in place (the x[], h, m, r bookkeeping of
+ * integer_partition.cpp:41-105); partitions with more than s parts are skipped.
+ * The same bookkeeping is restated here. */
+int64_t oracle_partitions(int n, int s, int32_t* out)
+{
+    int64_t rows = 0;
+    int32_t* x = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n + 2));
+    for (int i = 1; i <= n; ++i)
+        x[i] = 1;
+    if (n <= s)
+        emit_partition(x, n, s, out, &rows); /* all ones (integer_partition.cpp:58-62) */
+    if (n > 1) {
+        int h = 1, m = n - 1;
+        x[0] = -1;
+        x[1] = 2;
+        if (m <= s)
+            emit_partition(x, m, s, out, &rows);
+        while (x[1] != n) {
+            if (m - h > 1) { /* split: append another 2 (integer_partition.cpp:76-80) */
+                h += 1;
+                x[h] = 2;
+                m -= 1;
+            } else { /* carry (integer_partition.cpp:80-94) */
+                int j = m - 2;
+                while (x[j] == x[m - 1]) {
+                    x[j] = 1;
+                    j -= 1;
+                }
+                h = j + 1;
+                x[h] = x[m - 1] + 1;
+                int r = x[m] + x[m - 1] * (m - h - 1);
+                x[m] = 1;
+                if (m - h > 1)
+                    x[m - 1] = 1;
+                m = h + r - 1;
+            }
+            if (m <= s)
+                emit_partition(x, m, s, out, &rows);
+        }
+    }
+    free(x);
+    return rows;
+}
+
+/* ======================================================================== */
+/* hierarchy graph (src/heom/hierarchy_graph.cpp:30-181)                     */
+/* ======================================================================== */
+
+/* binomial coefficient evaluated in double and rounded, as the reference does with
+ * boost::math::binomial_coefficient<double> + llround (hierarchy_graph.cpp:34,40) */
+static int64_t binom_rounded(int n, int k)
+{
+    if (k < 0 || k > n)
+        return 0;
+    if (k > n - k)
+        k = n - k;
+    long double v = 1.0L;
+    for (int i = 1; i <= k; ++i)
+        v = v * (long double)(n - k + i) / (long double)i;
+    return (int64_t)llroundl(v);
+}
+
+int64_t oracle_graph_nodes(int width, int depth) { return binom_rounded(width + depth, width); }
+
+int64_t oracle_graph_first_non_plus(int width, int depth)
+{
+    return binom_rounded(width + depth - 1, width);
+}
+
+/* lexicographic successor of a[0..n) (same contract as std::next_permutation):
+ * returns 0 and leaves a sorted ascending when a was the last permutation */
+static int next_permutation_i32(int32_t* a, int n)
+{
+    int i = n - 2;
+    while (i >= 0 && a[i] >= a[i + 1])
+        --i;
+    if (i < 0) {
+        for (int l = 0, r = n - 1; l < r; ++l, --r) {
+            int32_t t = a[l];
+            a[l] = a[r];
+            a[r] = t;
+        }
+        return 0;
+    }
+    int j = n - 1;
+    while (a[j] <= a[i])
+        --j;
+    int32_t t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+    for (int l = i + 1, r = n - 1; l < r; ++l, --r) {
+        t = a[l];
+        a[l] = a[r];
+        a[r] = t;
+    }
+    return 1;
+}
+
+static int cmp_i32_asc(const void* a, const void* b)
+{
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* open-addressing map tuple -> uid (the reference uses an unordered_map,
+ * hierarchy_graph.cpp:49) */
+typedef struct {
+    int64_t cap;
+    int width;
+    int64_t* slot; /* uid or -1 */
+    const int32_t* tuples;
+} tuple_map;
+
+static uint64_t hash_tuple(const int32_t* t, int w)
+{
+    uint64_t h = 1469598103934665603ULL;
+    for (int i = 0; i < w; ++i) {
+        h ^= (uint64_t)(uint32_t)t[i];
+        h *= 1099511628211ULL;
+        h ^= h >> 29;
+    }
+    return h;
+}
+
+static int64_t map_find(const tuple_map* mp, const int32_t* key)
+{
+    uint64_t h = hash_tuple(key, mp->width) & (uint64_t)(mp->cap - 1);
+    for (;;) {
+        int64_t u = mp->slot[h];
+        if (u < 0)
+            return -1;
+        if (memcmp(mp->tuples + u * mp->width, key, sizeof(int32_t) * (size_t)mp->width) == 0)
+            return u;
+        h = (h + 1) & (uint64_t)(mp->cap - 1);
+    }
+}
+
+static void map_insert(tuple_map* mp, int64_t uid)
+{
+    const int32_t* key = mp->tuples + uid * mp->width;
+    uint64_t h = hash_tuple(key, mp->width) & (uint64_t)(mp->cap - 1);
+    while (mp->slot[h] >= 0)
+        h = (h + 1) & (uint64_t)(mp->cap - 1);
+    mp->slot[h] = uid;
+}
+
+int oracle_graph_build(int width, int depth, int32_t* tuples, int32_t* plus, int32_t* minus,
+                       int64_t* nodes_per_depth)
+{
+    const int64_t nodes = oracle_graph_nodes(width, depth);
+    const int32_t invalid = -2; /* hierarchy_graph.cpp:24 */
+    int64_t uid = 0;
+
+    /* Step 1: nodes layer by layer; inside a layer, partitions in generator order,
+     * each sorted ascending and expanded into all of its permutations
+     * (hierarchy_graph.cpp:57-117) */
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * (size_t)width);
+    for (int d = 0; d <= depth; ++d) {
+        int64_t count_d = 0;
+        if (d == 0) {
+            memset(tuples, 0, sizeof(int32_t) * (size_t)width);
+            uid = 1;
+            count_d = 1;
+        } else {
+            int64_t np = oracle_partition_count(d, width);
+            int32_t* parts = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np * width));
+            int64_t got = oracle_partitions(d, width, parts);
+            if (got != np) {
+                free(parts);
+                free(cur);
+                return -1;
+            }
+            for (int64_t p = 0; p < np; ++p) {
+                memcpy(cur, parts + p * width, sizeof(int32_t) * (size_t)width);
+                qsort(cur, (size_t)width, sizeof(int32_t), cmp_i32_asc);
+                do {
+                    if (uid >= nodes) {
+                        free(parts);
+                        free(cur);
+                        return -2;
+                    }
+                    memcpy(tuples + uid * width, cur, sizeof(int32_t) * (size_t)width);
+                    ++uid;
+                    ++count_d;
+                } while (next_permutation_i32(cur, width));
+            }
+            free(parts);
+        }
+        if (nodes_per_depth)
+            nodes_per_depth[d] = count_d;
+    }
+    free(cur);
+    if (uid != nodes)
+        return -3;
+
+    /* Step 2: plus / minus edges by tuple lookup (hierarchy_graph.cpp:122-168) */
+    tuple_map mp;
+    mp.width = width;
+    mp.tuples = tuples;
+    mp.cap = 1;
+    while (mp.cap < 2 * nodes + 16)
+        mp.cap <<= 1;
+    mp.slot = (int64_t*)malloc(sizeof(int64_t) * (size_t)mp.cap);
+    for (int64_t i = 0; i < mp.cap; ++i)
+        mp.slot[i] = -1;
+    for (int64_t u = 0; u < nodes; ++u)
+        map_insert(&mp, u);
+
+    int err = 0;
+#pragma omp parallel
+    {
+        int32_t* probe = (int32_t*)malloc(sizeof(int32_t) * (size_t)width);
+#pragma omp for schedule(static)
+        for (int64_t u = 0; u < nodes; ++u) {
+            const int32_t* t = tuples + u * width;
+            int dep = 0;
+            for (int m = 0; m < width; ++m)
+                dep += t[m];
+            for (int m = 0; m < width; ++m) {
+                memcpy(probe, t, sizeof(int32_t) * (size_t)width);
+                int32_t pu = invalid, mu = invalid;
+                if (dep < depth) {
+                    probe[m] += 1;
+                    int64_t f = map_find(&mp, probe);
+                    if (f < 0)
+                        err = 1;
+                    pu = (int32_t)f;
+                    probe[m] -= 1;
+                }
+                if (t[m] - 1 >= 0) {
+                    probe[m] -= 1;
+                    int64_t f = map_find(&mp, probe);
+                    if (f < 0)
+                        err = 1;
+                    mu = (int32_t)f;
+                }
+                plus[u * width + m] = pu;
+                minus[u * width + m] = mu;
+            }
+        }
+        free(probe);
+    }
+    free(mp.slot);
+    return err ? -4 : 0;
+}
+
+/* ======================================================================== */
+/* precompute (src/heom/instance.cpp:155-262)                                */
+/* ======================================================================== */
+
+void oracle_precompute(int baths, int matsubaras, const double* lambda, const double* invnu,
+                       const double* omega, double temperature, int64_t matrices, int width,
+                       const int32_t* tuples, double* pf_same_sum, double* pf_same,
+                       double* pf_circ, double* cbk_out, double* cbk_sqrt)
+{
+    const double hbar = ORACLE_HBAR;
+    const double beta = 1.0 / (ORACLE_KB * temperature); /* instance.cpp:167 */
+    const int row = (matsubaras - 1) * ORACLE_MAX_PADE;
+
+    for (int b = 0; b < baths; ++b) {
+        double complex lambda_b = CMPLX(lambda[b], 0.0);
+        double complex nu_b = CMPLX(1.0 / invnu[b], omega[b]); /* instance.cpp:174 */
+        double complex circ = nu_b * lambda_b / hbar;          /* instance.cpp:176 */
+        double complex aux = 2.0 * lambda_b / (beta * hbar * hbar * nu_b); /* :177 */
+
+        for (int k = 0; k < matsubaras; ++k) {
+            double complex c, nubk;
+            if (k == 0) { /* Drude term with the Pade correction (instance.cpp:183-200) */
+                double complex auxsum = 0.0;
+                for (int j = 0; j < matsubaras; ++j) {
+                    double eta = (double)oracle_eta_pade[row + j];
+                    double xi = (double)oracle_xi_pade[row + j];
+                    auxsum += (2.0 * eta * nu_b * nu_b) /
+                              (xi * xi / (beta * beta * hbar * hbar) - nu_b * nu_b);
+                }
+                double complex c0 = 2.0 * lambda_b / (beta * hbar) * (1.0 - auxsum);
+                c = c0;
+                nubk = nu_b;
+            } else { /* Pade pole k (instance.cpp:202-214) */
+                double eta = (double)oracle_eta_pade[row + k];
+                double xi = (double)oracle_xi_pade[row + k];
+                c = 4.0 * lambda_b * nu_b / (beta * beta * hbar * hbar) * (eta * xi) /
+                    (xi * xi / (beta * beta * hbar * hbar) - nu_b * nu_b);
+                nubk = xi / (beta * hbar);
+            }
+            cbk_out[2 * (b * matsubaras + k)] = creal(c);      /* instance.cpp:216 */
+            cbk_out[2 * (b * matsubaras + k) + 1] = cimag(c);
+            cbk_sqrt[b * matsubaras + k] = sqrt(cabs(c));      /* instance.cpp:217 */
+            aux -= c / (hbar * nubk);                           /* instance.cpp:218 */
+        }
+        pf_same[2 * b] = creal(aux);
+        pf_same[2 * b + 1] = cimag(aux);
+        pf_circ[2 * b] = creal(circ);
+        pf_circ[2 * b + 1] = cimag(circ);
+    }
+
+    /* S_n = -sum_{b,k} n_bk nu_bk in long double (instance.cpp:227-262) */
+#pragma omp parallel for schedule(static)
+    for (int64_t id = 0; id < matrices; ++id) {
+        long double zr = 0.0L, zi = 0.0L;
+        for (int b = 0; b < baths; ++b) {
+            double complex nu_b = CMPLX(1.0 / invnu[b], omega[b]);
+            for (int k = 0; k < matsubaras; ++k) {
+                long double nr, ni;
+                if (k == 0) {
+                    nr = (long double)creal(nu_b);
+                    ni = (long double)cimag(nu_b);
+                } else {
+                    nr = oracle_xi_pade[row + k] / (long double)(beta * hbar);
+                    ni = 0.0L;
+                }
+                int m = b * matsubaras + k;
+                zr -= nr * (long double)tuples[id * width + m];
+                zi -= ni * (long double)tuples[id * width + m];
+            }
+        }
+        pf_same_sum[2 * id] = (double)zr;
+        pf_same_sum[2 * id + 1] = (double)zi;
+    }
+}
+
+/* ======================================================================== */
+/* RHS (cl/heom/heom_ode.cl:24-451)                                          */
+/* ======================================================================== */
+
+void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
+{
+    const int N = inst->states;
+    const int K = inst->matsubaras;
+    const int BM = inst->baths_max_per_state;
+    const int W = inst->width;
+    const int64_t M = inst->matrices;
+    const double HB = ORACLE_HBAR;
+    const double* H = inst->hamiltonian;
+
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t id = 0; id < M; ++id) {
+        const double* s = in + id * N * N * 2;
+        double* o = out + id * N * N * 2;
+        for (int i = 0; i < N; ++i) {
+            for (int j = 0; j < N; ++j) {
+                const double s_r = s[(i * N + j) * 2];
+                const double s_i = s[(i * N + j) * 2 + 1];
+
+                /* commutator (i/ħ)(σH - Hσ) with separate temporaries (:78-114) */
+                double com_r = 0.0, com_i = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    double sh_r = s[(i * N + k) * 2] * H[(k * N + j) * 2];
+                    sh_r -= s[(i * N + k) * 2 + 1] * H[(k * N + j) * 2 + 1];
+                    double sh_i = s[(i * N + k) * 2] * H[(k * N + j) * 2 + 1];
+                    sh_i += s[(i * N + k) * 2 + 1] * H[(k * N + j) * 2];
+                    double hs_r = H[(i * N + k) * 2] * s[(k * N + j) * 2];
+                    hs_r -= H[(i * N + k) * 2 + 1] * s[(k * N + j) * 2 + 1];
+                    double hs_i = H[(i * N + k) * 2] * s[(k * N + j) * 2 + 1];
+                    hs_i += H[(i * N + k) * 2 + 1] * s[(k * N + j) * 2];
+                    com_r += sh_r - hs_r;
+                    com_i += sh_i - hs_i;
+                }
+                {
+                    double t = com_r;
+                    com_r = -com_i / HB;
+                    com_i = t / HB;
+                }
+
+                /* same term: S_n σ_ij and -Δ_b for the baths of i and of j (:126-169) */
+                const double pss_r = inst->pf_same_sum[2 * id];
+                const double pss_i = inst->pf_same_sum[2 * id + 1];
+                double same_r = pss_r * s_r - pss_i * s_i;
+                double same_i = pss_r * s_i + pss_i * s_r;
+                if (i != j) {
+                    for (int b = 0; b < BM; ++b) {
+                        const int bi = inst->coupling[i * BM + b];
+                        if (bi >= 0) {
+                            const double mr = -inst->pf_same[2 * bi];
+                            const double mi = -inst->pf_same[2 * bi + 1];
+                            same_r += mr * s_r - mi * s_i;
+                            same_i += mr * s_i + mi * s_r;
+                        }
+                        const int bj = inst->coupling[j * BM + b];
+                        if (bj >= 0) {
+                            const double mr = -inst->pf_same[2 * bj];
+                            const double mi = -inst->pf_same[2 * bj + 1];
+                            same_r += mr * s_r - mi * s_i;
+                            same_i += mr * s_i + mi * s_r;
+                        }
+                    }
+                }
+
+                /* plus term, separate i and j accumulators (:172-266) */
+                double pl_ir = 0.0, pl_ii = 0.0, pl_jr = 0.0, pl_ji = 0.0;
+                if (id < inst->first_non_plus) {
+                    for (int b = 0; b < BM; ++b) {
+                        const int bi = inst->coupling[i * BM + b];
+                        if (bi >= 0) {
+                            for (int m = 0; m < K; ++m) {
+                                const int mm = bi * K + m;
+                                const int32_t pu = inst->plus[id * W + mm];
+                                if (!inst->mask || inst->mask[pu]) {
+                                    const double* sp = in + (int64_t)pu * N * N * 2;
+                                    const double sp_r = sp[(i * N + j) * 2];
+                                    const double sp_i = sp[(i * N + j) * 2 + 1];
+                                    const int t = inst->tuples[id * W + mm];
+                                    const double a_r = 0.0;
+                                    const double a_i =
+                                        sqrt((t + 1.0) * hypot(inst->cbk[2 * mm], inst->cbk[2 * mm + 1]) / HB);
+                                    pl_ir += a_r * sp_r - a_i * sp_i;
+                                    pl_ii += a_r * sp_i + a_i * sp_r;
+                                }
+                            }
+                        }
+                        const int bj = inst->coupling[j * BM + b];
+                        if (bj >= 0) {
+                            for (int m = 0; m < K; ++m) {
+                                const int mm = bj * K + m;
+                                const int32_t pu = inst->plus[id * W + mm];
+                                if (!inst->mask || inst->mask[pu]) {
+                                    const double* sp = in + (int64_t)pu * N * N * 2;
+                                    const double sp_r = sp[(i * N + j) * 2];
+                                    const double sp_i = sp[(i * N + j) * 2 + 1];
+                                    const int t = inst->tuples[id * W + mm];
+                                    const double a_r = 0.0;
+                                    const double a_i =
+                                        sqrt((t + 1.0) * hypot(inst->cbk[2 * mm], inst->cbk[2 * mm + 1]) / HB);
+                                    pl_jr -= a_r * sp_r - a_i * sp_i;
+                                    pl_ji -= a_r * sp_i + a_i * sp_r;
+                                }
+                            }
+                        }
+                    }
+                }
+                const double plus_r = pl_ir + pl_jr;
+                const double plus_i = pl_ii + pl_ji;
+
+                /* minus term (:269-409) */
+                double mn_ir = 0.0, mn_ii = 0.0, mn_jr = 0.0, mn_ji = 0.0;
+                for (int b = 0; b < BM; ++b) {
+                    const int bi = inst->coupling[i * BM + b];
+                    if (bi >= 0) {
+                        for (int m = 0; m < K; ++m) {
+                            const int mm = bi * K + m;
+                            const int t = inst->tuples[id * W + mm];
+                            if (t > 0) {
+                                const int32_t mu = inst->minus[id * W + mm];
+                                if (!inst->mask || inst->mask[mu]) {
+                                    const double* sm = in + (int64_t)mu * N * N * 2;
+                                    const double sm_r = sm[(i * N + j) * 2];
+                                    const double sm_i = sm[(i * N + j) * 2 + 1];
+                                    const double c_r = inst->cbk[2 * mm];
+                                    const double c_i = inst->cbk[2 * mm + 1];
+                                    const double pc_r = sm_r * c_r - sm_i * c_i;
+                                    const double pc_i = sm_r * c_i + sm_i * c_r;
+                                    const double a_r = -pc_i / HB;
+                                    const double a_i = pc_r / HB;
+                                    const double sc = sqrt(t / hypot(c_r, c_i) * HB);
+                                    mn_ir += a_r * sc;
+                                    mn_ii += a_i * sc;
+                                    if (m == 0) {
+                                        const double f_r = inst->pf_circ[2 * bi];
+                                        const double f_i = inst->pf_circ[2 * bi + 1];
+                                        mn_ir += (sm_r * f_r - sm_i * f_i) * sc;
+                                        mn_ii += (sm_r * f_i + sm_i * f_r) * sc;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    const int bj = inst->coupling[j * BM + b];
+                    if (bj >= 0) {
+                        for (int m = 0; m < K; ++m) {
+                            const int mm = bj * K + m;
+                            const int t = inst->tuples[id * W + mm];
+                            if (t > 0) {
+                                const int32_t mu = inst->minus[id * W + mm];
+                                if (!inst->mask || inst->mask[mu]) {
+                                    const double* sm = in + (int64_t)mu * N * N * 2;
+                                    const double sm_r = sm[(i * N + j) * 2];
+                                    const double sm_i = sm[(i * N + j) * 2 + 1];
+                                    const double c_r = inst->cbk[2 * mm];
+                                    const double c_i = inst->cbk[2 * mm + 1];
+                                    const double pc_r = sm_r * c_r - sm_i * c_i;
+                                    const double pc_i = sm_r * c_i + sm_i * c_r;
+                                    const double a_r = pc_i / HB;
+                                    const double a_i = -pc_r / HB;
+                                    const double sc = sqrt(t / hypot(c_r, c_i) * HB);
+                                    mn_jr += a_r * sc;
+                                    mn_ji += a_i * sc;
+                                    if (m == 0) {
+                                        const double f_r = inst->pf_circ[2 * bj];
+                                        const double f_i = inst->pf_circ[2 * bj + 1];
+                                        mn_jr += (sm_r * f_r - sm_i * f_i) * sc;
+                                        mn_ji += (sm_r * f_i + sm_i * f_r) * sc;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                const double minus_r = mn_ir + mn_jr;
+                const double minus_i = mn_ii + mn_ji;
+
+                /* sum of the four groups (:413-414), pure evaluation (:432-433) */
+                o[(i * N + j) * 2] = com_r + same_r + plus_r + minus_r;
+                o[(i * N + j) * 2 + 1] = com_i + same_i + plus_i + minus_i;
+            }
+        }
+    }
+}
+
+/* ======================================================================== */
+/* RK (cl/heom/rk_weighted_add.cl:19-106 and the classical RK4 tableau)      */
+/* ======================================================================== */
+
+void oracle_rk_weighted_add(int64_t size, int n, double h, double* out, const double* y_n,
+                            const double* c, const double* const* k)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < size; ++e) {
+        double v = 0.0;
+        for (int l = 0; l < n; ++l)
+            if (c[l] != 0.0)
+                v += c[l] * k[l][e];
+        v *= h;
+        v += y_n[e];
+        out[e] = v;
+    }
+}
+
+/* Classical RK4 (Butcher a = [[1/2], [0, 1/2], [0, 0, 1]], b = [1/6, 1/3, 1/3, 1/6]).
+ * The reference drives it through the un-vendored noma/num stepper
+ * (include/heom/population_dynamics_solver.hpp:112-114, 242); here every stage input
+ * and the final combination go through rk_weighted_add as written above. */
+void oracle_rk4_step(const oracle_instance* inst, double h, double* y, double* work)
+{
+    const int64_t size = inst->matrices * inst->states * inst->states * 2;
+    double* k1 = work;
+    double* k2 = work + size;
+    double* k3 = work + 2 * size;
+    double* k4 = work + 3 * size;
+    double* t = work + 4 * size;
+    const double* ks[4] = {k1, k2, k3, k4};
+    const double a2[1] = {0.5};
+    const double a3[2] = {0.0, 0.5};
+    const double a4[3] = {0.0, 0.0, 1.0};
+    const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+    oracle_rhs(inst, y, k1);
+    oracle_rk_weighted_add(size, 1, h, t, y, a2, ks);
+    oracle_rhs(inst, t, k2);
+    oracle_rk_weighted_add(size, 2, h, t, y, a3, ks);
+    oracle_rhs(inst, t, k3);
+    oracle_rk_weighted_add(size, 3, h, t, y, a4, ks);
+    oracle_rhs(inst, t, k4);
+    oracle_rk_weighted_add(size, 4, h, t, y, b, ks);
+    memcpy(y, t, sizeof(double) * (size_t)size);
+}
+
+int64_t oracle_propagate(const oracle_instance* inst, double h, int64_t nsteps,
+                         int64_t observe_steps, double* y, double* work, double* tops)
+{
+    const int64_t top = (int64_t)inst->states * inst->states * 2;
+    int64_t rows = 0;
+    if (tops)
+        memcpy(tops + (rows++) * top, y, sizeof(double) * (size_t)top);
+    for (int64_t s = 1; s <= nsteps; ++s) {
+        oracle_rk4_step(inst, h, y, work);
+        if (tops && observe_steps > 0 && s % observe_steps == 0)
+            memcpy(tops + (rows++) * top, y, sizeof(double) * (size_t)top);
+    }
+    return rows;
+}
+
+/* ======================================================================== */
+/* observers and hierarchy arithmetic                                        */
+/* ======================================================================== */
+
+void oracle_hierarchy_norm(int states, int64_t matrices, const double* sigma, double threshold,
+                           double* norms, uint8_t* mask)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t id = 0; id < matrices; ++id) {
+        const double* a = sigma + id * states * states * 2;
+        double acc = 0.0;
+        for (int e = 0; e < states * states; ++e)
+            acc += a[2 * e] * a[2 * e] + a[2 * e + 1] * a[2 * e + 1];
+        norms[id] = acc;
+        if (mask && acc - threshold <= 0.0 && mask[id] != 2)
+            mask[id] = 0;
+    }
+}
+
+void oracle_mmult_left(int states, int64_t matrices, const double* mx, double* sigma)
+{
+    const int N = states;
+#pragma omp parallel
+    {
+        double* tmp = (double*)malloc(sizeof(double) * (size_t)(N * N * 2));
+#pragma omp for schedule(static)
+        for (int64_t id = 0; id < matrices; ++id) {
+            double* a = sigma + id * N * N * 2;
+            for (int i = 0; i < N; ++i) {
+                for (int j = 0; j < N; ++j) {
+                    double vr = 0.0, vi = 0.0;
+                    for (int k = 0; k < N; ++k) {
+                        const double mr = mx[(i * N + k) * 2], mi = mx[(i * N + k) * 2 + 1];
+                        const double ar = a[(k * N + j) * 2], ai = a[(k * N + j) * 2 + 1];
+                        const double pr = mr * ar - mi * ai;
+                        const double pi = mr * ai + mi * ar;
+                        vr = vr + pr;
+                        vi = vi + pi;
+                    }
+                    tmp[(i * N + j) * 2] = vr;
+                    tmp[(i * N + j) * 2 + 1] = vi;
+                }
+            }
+            memcpy(a, tmp, sizeof(double) * (size_t)(N * N * 2));
+        }
+        free(tmp);
+    }
+}
+
+void oracle_trace(int states, const double* d, const double* top, double* trace2)
+{
+    const int N = states;
+    double tr = 0.0, ti = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double vr, vi;
+        if (!d) {
+            vr = top[(i * N + i) * 2];
+            vi = top[(i * N + i) * 2 + 1];
+        } else {
+            vr = 0.0;
+            vi = 0.0;
+            for (int k = 0; k < N; ++k) {
+                const double dr = d[(i * N + k) * 2], di = d[(i * N + k) * 2 + 1];
+                const double rr = top[(k * N + i) * 2], ri = top[(k * N + i) * 2 + 1];
+                vr += dr * rr - di * ri;
+                vi += dr * ri + di * rr;
+            }
+        }
+        tr += vr;
+        ti += vi;
+    }
+    trace2[0] = tr;
+    trace2[1] = ti;
+}
