@@ -1,30 +1,10 @@
-// heom_hip.hip -- MI355X (gfx950) HEOM solver: device kernels and the C ABI of
-// include/heom_hip.h.
-//
-// Device layout (one hierarchy, FP64):
-//   * ADOs are renumbered into lexicographic order of their tuples (mode 0 most
-//     significant).  Neighbours n +- e_m then sit close in index: at 735k ADOs four
-//     fifths of all neighbour reads land within 30k ADOs, against one in twenty in the
-//     reference's layer order, so most of them are served from L2 / MALL.
-//   * State planes are structure-of-arrays by matrix element: Y[e * Mp + q] holds
-//     element e = i*N + j of ADO q as a double2 (re, im).  Mp = M rounded up to a
-//     multiple of the 64-ADO tile; padding ADOs have no neighbours and stay zero.
-//   * Neighbour tables are SoA by mode: plus[m * Mp + q], minus[m * Mp + q] (int32,
-//     -1 = no edge or neighbour masked), tup[m * Mp + q] (uint8, n_m).
-//
-// Stage kernel (rk4_stage): one workgroup = one tile of 64 ADOs, one wavefront per
-// matrix row i (N wavefronts), lane = ADO.  Each lane loads row i of its ADO (one
-// coalesced 1 KiB load per element plane), publishes it to an LDS copy of the tile,
-// and computes row i of f(sigma): the commutator (row i of sigma from registers, the
-// columns from LDS, H/hbar through scalar loads), the "same" term, and the neighbour
-// terms as coalesced gathers from the element planes.  The classical RK4 combination
-// of the stage is applied in the epilogue, so a step is four launches and 16 state
-// passes (R y, W Y2, W A | R Y2, R y, R A, W Y3, W A | ... | R Y4, R A, W y).
+// heom_hip.hip -- the C ABI of include/heom_hip.h: handle lifetime, the re-layout of
+// the reference instance into the device layout (heom_device.hpp), the RK4 driver,
+// and the small layout / reduction / hierarchy-arithmetic kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -34,14 +14,14 @@
 #include <string>
 #include <vector>
 
+#include "heom_device.hpp"
 #include "heom_hip.h"
+
+using namespace heom_hip;
 
 namespace {
 
-constexpr int kTile = 64;      // ADOs per workgroup (one per lane)
-constexpr int kMaxStates = 8;  // N handled by the stage kernel
-constexpr int kNumXcd = 8;     // MI355X: 8 XCDs, workgroups dispatched round-robin
-constexpr double kHbar = 1.05457148e-34;  // cl/heom/common.cl:45
+constexpr double kHbar = 1.05457148e-34;  // dm-heom/cl/heom/common.cl:45
 
 thread_local std::string g_error;
 
@@ -60,215 +40,10 @@ void hip_check(hipError_t e, const char* what)
 #define HIPC(x) hip_check((x), #x)
 
 // ------------------------------------------------------------------------------------
-// complex helpers (double2 = re, im)
-// ------------------------------------------------------------------------------------
-
-__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c)  // c + a*b
-{
-    c.x = fma(a.x, b.x, c.x);
-    c.x = fma(-a.y, b.y, c.x);
-    c.y = fma(a.x, b.y, c.y);
-    c.y = fma(a.y, b.x, c.y);
-    return c;
-}
-
-__device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c)  // c - a*b
-{
-    c.x = fma(-a.x, b.x, c.x);
-    c.x = fma(a.y, b.y, c.x);
-    c.y = fma(-a.x, b.y, c.y);
-    c.y = fma(-a.y, b.x, c.y);
-    return c;
-}
-
-// workgroup index -> tile index so that each XCD works on one contiguous range of
-// tiles (neighbour reads then hit that XCD's L2)
-__device__ __forceinline__ int xcd_tile(int b, int nt)
-{
-    const int per = nt / kNumXcd, rem = nt % kNumXcd;
-    const int x = b % kNumXcd, k = b / kNumXcd;
-    return x * per + min(x, rem) + k;
-}
-
-// ------------------------------------------------------------------------------------
-// stage kernel
-// ------------------------------------------------------------------------------------
-
-struct StageArgs {
-    const double2* __restrict__ yin;   // stage input Y_s (neighbours are read from here)
-    const double2* __restrict__ y0;    // y_n
-    double2* __restrict__ acc;         // RK accumulator A
-    double2* __restrict__ yout;        // next stage input, or y_{n+1}, or f
-    const int32_t* __restrict__ plus;  // [W][Mp]
-    const int32_t* __restrict__ minus; // [W][Mp]
-    const uint8_t* __restrict__ tup;   // [W][Mp]
-    const double2* __restrict__ pfss;  // [Mp] pf_same_sum
-    const double2* __restrict__ hs;    // [N*N] H / hbar
-    const double2* __restrict__ gam;   // [N*N] -(sum of Delta_b over the baths of i and j), i != j
-    const int32_t* __restrict__ rowm_cnt; // [N]      modes of the baths of state i
-    const int32_t* __restrict__ rowm;     // [N][W]
-    const int32_t* __restrict__ colm_cnt; // [N*N]    modes of the baths of j not in row i's list
-    const int32_t* __restrict__ colm;     // [N*N][W]
-    const double* __restrict__ pm;        // [N*N][W] count_i(m) - count_j(m)
-    const double2* __restrict__ beta;     // [N*N][W] minus-term constant of (e, m)
-    const double* __restrict__ pcoef;     // [W][nv] sqrt((v+1)|c_m|/hbar)
-    const double* __restrict__ scoef;     // [W][nv] sqrt(v/|c_m| hbar)
-    int64_t mp;
-    int32_t width;
-    int32_t nv;
-    int32_t ntiles;
-    double c_out;  // coefficient of f in yout
-    double c_acc;  // coefficient of f in acc
-};
-
-// STAGE 0: yout = f(yin).  STAGE 1..4: classical RK4 stage s (see file header).
-template <int N, int STAGE, bool REALH>
-__global__ __launch_bounds__(64 * N) void rk4_stage(StageArgs a)
-{
-    __shared__ double2 tile_lds[N * N * kTile];
-    const int lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = xcd_tile(blockIdx.x, a.ntiles);
-    const int64_t mp = a.mp;
-    const int64_t q = (int64_t)tile * kTile + lane;
-    const int W = a.width;
-
-    double2 s[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        s[j] = a.yin[(int64_t)(i * N + j) * mp + q];
-        tile_lds[(i * N + j) * kTile + lane] = s[j];
-    }
-    __syncthreads();
-
-    // commutator c = sigma Hs - Hs sigma (row i); f = i c
-    double2 c[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-        c[j] = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const double2 h = a.hs[k * N + j];
-            if (REALH) {
-                c[j].x = fma(s[k].x, h.x, c[j].x);
-                c[j].y = fma(s[k].y, h.x, c[j].y);
-            } else {
-                c[j] = cfma(s[k], h, c[j]);
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double2 h = a.hs[i * N + k];
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const double2 skj = tile_lds[(k * N + j) * kTile + lane];
-            if (REALH) {
-                c[j].x = fma(-h.x, skj.x, c[j].x);
-                c[j].y = fma(-h.x, skj.y, c[j].y);
-            } else {
-                c[j] = cfms(h, skj, c[j]);
-            }
-        }
-    }
-    double2 f[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-        f[j] = make_double2(-c[j].y, c[j].x);
-
-    // same term: (S_n + gamma_ij) sigma_ij
-    const double2 S = a.pfss[q];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const double2 g = a.gam[i * N + j];
-        const double2 t = make_double2(S.x + g.x, S.y + g.y);
-        f[j] = cfma(t, s[j], f[j]);
-    }
-
-    // neighbour terms of the modes of state i's baths (present in every column j)
-    const int nr = a.rowm_cnt[i];
-    for (int r = 0; r < nr; ++r) {
-        const int m = __builtin_amdgcn_readfirstlane(a.rowm[i * W + r]);
-        const int32_t p = a.plus[(int64_t)m * mp + q];
-        const int32_t mi = a.minus[(int64_t)m * mp + q];
-        const int v = a.tup[(int64_t)m * mp + q];
-        const double pf = a.pcoef[m * a.nv + v];
-        const double sf = a.scoef[m * a.nv + v];
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const int e = i * N + j;
-            const double pmul = a.pm[e * W + m];
-            const double2 bt = a.beta[e * W + m];
-            if (pmul != 0.0 && p >= 0) {
-                const double2 x = a.yin[(int64_t)e * mp + p];
-                const double cf = pmul * pf;
-                f[j].x = fma(-cf, x.y, f[j].x);
-                f[j].y = fma(cf, x.x, f[j].y);
-            }
-            if ((bt.x != 0.0 || bt.y != 0.0) && mi >= 0) {
-                const double2 x = a.yin[(int64_t)e * mp + mi];
-                f[j] = cfma(make_double2(bt.x * sf, bt.y * sf), x, f[j]);
-            }
-        }
-    }
-    // neighbour terms of the modes of state j's baths that state i does not couple to
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int e = i * N + j;
-        const int nc = a.colm_cnt[e];
-        for (int cidx = 0; cidx < nc; ++cidx) {
-            const int m = __builtin_amdgcn_readfirstlane(a.colm[e * W + cidx]);
-            const double pmul = a.pm[e * W + m];
-            const double2 bt = a.beta[e * W + m];
-            const int v = a.tup[(int64_t)m * mp + q];
-            if (pmul != 0.0) {
-                const int32_t p = a.plus[(int64_t)m * mp + q];
-                if (p >= 0) {
-                    const double2 x = a.yin[(int64_t)e * mp + p];
-                    const double cf = pmul * a.pcoef[m * a.nv + v];
-                    f[j].x = fma(-cf, x.y, f[j].x);
-                    f[j].y = fma(cf, x.x, f[j].y);
-                }
-            }
-            if (bt.x != 0.0 || bt.y != 0.0) {
-                const int32_t mi = a.minus[(int64_t)m * mp + q];
-                if (mi >= 0) {
-                    const double sf = a.scoef[m * a.nv + v];
-                    const double2 x = a.yin[(int64_t)e * mp + mi];
-                    f[j] = cfma(make_double2(bt.x * sf, bt.y * sf), x, f[j]);
-                }
-            }
-        }
-    }
-
-    // RK4 epilogue
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int64_t idx = (int64_t)(i * N + j) * mp + q;
-        if (STAGE == 0) {
-            a.yout[idx] = f[j];
-        } else if (STAGE == 1) {
-            a.yout[idx] = make_double2(fma(a.c_out, f[j].x, s[j].x), fma(a.c_out, f[j].y, s[j].y));
-            a.acc[idx] = make_double2(fma(a.c_acc, f[j].x, s[j].x), fma(a.c_acc, f[j].y, s[j].y));
-        } else if (STAGE == 2 || STAGE == 3) {
-            const double2 y = a.y0[idx];
-            const double2 A = a.acc[idx];
-            a.yout[idx] = make_double2(fma(a.c_out, f[j].x, y.x), fma(a.c_out, f[j].y, y.y));
-            a.acc[idx] = make_double2(fma(a.c_acc, f[j].x, A.x), fma(a.c_acc, f[j].y, A.y));
-        } else {
-            const double2 A = a.acc[idx];
-            a.yout[idx] = make_double2(fma(a.c_acc, f[j].x, A.x), fma(a.c_acc, f[j].y, A.y));
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------
 // layout, reduction and hierarchy-arithmetic kernels
 // ------------------------------------------------------------------------------------
 
-// dst[e * mp + q] = src[u(q)][e]   (reference AoS -> internal SoA)
+// dst[e * mp + q] = src[u(q)][e]   (reference AoS -> internal SoA; padding -> 0)
 __global__ void aos_to_soa(const double2* __restrict__ src, double2* __restrict__ dst,
                            const int32_t* __restrict__ q2u, int64_t mp, int nel)
 {
@@ -295,16 +70,16 @@ __global__ void soa_to_aos(const double2* __restrict__ src, double2* __restrict_
 }
 
 // dst[e] = src[e * mp + q0] for one ADO (the top), e < nel
-__global__ void copy_ado_out(const double2* __restrict__ src, double2* __restrict__ dst,
-                             int64_t q0, int64_t mp, int nel)
+__global__ void copy_ado_out(const double2* __restrict__ src, double2* __restrict__ dst, int64_t q0,
+                             int64_t mp, int nel)
 {
     const int e = threadIdx.x;
     if (e < nel)
         dst[e] = src[(int64_t)e * mp + q0];
 }
 
-__global__ void copy_ado_in(const double2* __restrict__ src, double2* __restrict__ dst,
-                            int64_t q0, int64_t mp, int nel)
+__global__ void copy_ado_in(const double2* __restrict__ src, double2* __restrict__ dst, int64_t q0,
+                            int64_t mp, int nel)
 {
     const int e = threadIdx.x;
     if (e < nel)
@@ -312,10 +87,10 @@ __global__ void copy_ado_in(const double2* __restrict__ src, double2* __restrict
 }
 
 // per-ADO squared Frobenius norm in uid order; threshold masking as in
-// cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
+// dm-heom/cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
 __global__ void hierarchy_norm_kernel(const double2* __restrict__ y, const int32_t* __restrict__ q2u,
-                                      int64_t mp, int nel, double threshold,
-                                      double* __restrict__ norms, uint8_t* __restrict__ mask)
+                                      int64_t mp, int nel, double threshold, double* __restrict__ norms,
+                                      uint8_t* __restrict__ mask)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= mp)
@@ -334,24 +109,26 @@ __global__ void hierarchy_norm_kernel(const double2* __restrict__ y, const int32
         mask[u] = 0;
 }
 
-// fold the uid-order mask into the neighbour tables: an edge whose target is masked
-// becomes -1 (the kernel then skips it, as heom_ode.cl does with its mask checks)
-__global__ void fold_mask(const int32_t* __restrict__ raw, int32_t* __restrict__ folded,
-                          const uint8_t* __restrict__ mask, const int32_t* __restrict__ q2u,
-                          int64_t count)
+// Fold the uid-order mask into the packed neighbour table: an edge whose target is
+// masked reads the zero slot (heom_ode.cl drops such a term with its mask checks).
+__global__ void fold_mask(const int2* __restrict__ raw, int2* __restrict__ folded, const uint8_t* __restrict__ mask,
+                          const int32_t* __restrict__ q2u, int64_t count, int32_t zero_slot)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count)
         return;
-    const int32_t p = raw[t];
-    folded[t] = (p >= 0 && mask[q2u[p]] != 0) ? p : -1;
+    const int2 r = raw[t];
+    const int32_t p = (int32_t)((uint32_t)r.x >> 6);
+    const int32_t n = r.x & 63;
+    const int32_t pf = (p != zero_slot && mask[q2u[p]] != 0) ? p : zero_slot;
+    const int32_t mf = (r.y != zero_slot && mask[q2u[r.y]] != 0) ? r.y : zero_slot;
+    folded[t] = make_int2((int32_t)(((uint32_t)pf << 6) | (uint32_t)n), mf);
 }
 
 // sigma_q <- Mx sigma_q for every ADO: wave = row i, lane = ADO, tile in LDS
 template <int N>
-__global__ __launch_bounds__(64 * N) void mmult_left_kernel(double2* __restrict__ y,
-                                                            const double2* __restrict__ mx,
-                                                            int64_t mp, int ntiles)
+__global__ __launch_bounds__(64 * N) void mmult_left_kernel(double2* __restrict__ y, const double2* __restrict__ mx,
+                                                            int64_t mp)
 {
     __shared__ double2 t[N * N * kTile];
     const int lane = threadIdx.x & 63;
@@ -378,7 +155,7 @@ __global__ __launch_bounds__(64 * N) void mmult_left_kernel(double2* __restrict_
 }
 
 // ------------------------------------------------------------------------------------
-// host side
+// device buffers
 // ------------------------------------------------------------------------------------
 
 template <typename T>
@@ -412,24 +189,24 @@ struct DevBuf {
 struct heom_handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    int N = 0, K = 0, B = 0, BM = 0, W = 0, nel = 0, nv = 0;
+    int N = 0, K = 0, B = 0, BM = 0, W = 0, nel = 0;
     int64_t M = 0, Mp = 0;
     int ntiles = 0;
+    int32_t zero_slot = 0;
     bool real_h = false;
-    int64_t q_top = 0;  // internal index of uid 0
+    int term_slots = 2;  // neighbour terms per element the stage kernel runs with
+    int64_t q_top = 0;   // internal index of uid 0
 
-    std::vector<int32_t> q2u_host;  // [Mp]
     std::vector<int32_t> coupling;  // [N * BM]
-    std::vector<double> cbk;        // [2 * B * K]
-    std::vector<double> pf_same;    // [2 * B]
-    std::vector<double> pf_circ;    // [2 * B]
+    std::vector<double> cbk, pf_same, pf_circ;
 
-    DevBuf<double2> y, acc, ya, yb;
-    DevBuf<int32_t> plus_raw, minus_raw, plus, minus, q2u;
-    DevBuf<uint8_t> tup, mask;
-    DevBuf<double2> pfss, hs, gam, beta, mx, small;
-    DevBuf<int32_t> rowm_cnt, rowm, colm_cnt, colm;
-    DevBuf<double> pm, pcoef, scoef, norms;
+    DevBuf<double2> y, pbuf, ya, yb;
+    DevBuf<int2> nb_raw, nb;  // packed neighbour table, unmasked and mask-folded
+    DevBuf<int32_t> q2u;
+    DevBuf<uint8_t> mask;
+    DevBuf<double2> pfss, hs, gam, mx, small;
+    DevBuf<Term> terms;
+    DevBuf<double> norms;
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timing_pending = false;
@@ -440,16 +217,6 @@ struct heom_handle {
     {
         if (stream)
             (void)hipStreamSynchronize(stream);
-        for (auto* b : {&y, &acc, &ya, &yb, &pfss, &hs, &gam, &beta, &mx, &small})
-            b->free();
-        for (auto* b : {&plus_raw, &minus_raw, &plus, &minus, &q2u, &rowm_cnt, &rowm, &colm_cnt, &colm})
-            b->free();
-        tup.free();
-        mask.free();
-        pm.free();
-        pcoef.free();
-        scoef.free();
-        norms.free();
         if (ev_start)
             (void)hipEventDestroy(ev_start);
         if (ev_stop)
@@ -482,96 +249,42 @@ void resolve_timing(heom_handle* h)
     h->pending_steps = 0;
 }
 
-StageArgs stage_args(heom_handle* h)
+void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, double c1)
 {
     StageArgs a{};
-    a.plus = h->plus.p;
-    a.minus = h->minus.p;
-    a.tup = h->tup.p;
-    a.pfss = h->pfss.p;
-    a.hs = h->hs.p;
-    a.gam = h->gam.p;
-    a.rowm_cnt = h->rowm_cnt.p;
-    a.rowm = h->rowm.p;
-    a.colm_cnt = h->colm_cnt.p;
-    a.colm = h->colm.p;
-    a.pm = h->pm.p;
-    a.beta = h->beta.p;
-    a.pcoef = h->pcoef.p;
-    a.scoef = h->scoef.p;
-    a.mp = h->Mp;
-    a.width = h->W;
-    a.nv = h->nv;
-    a.ntiles = h->ntiles;
-    return a;
-}
-
-template <int N, bool REALH>
-void launch_stage_n(int stage, const StageArgs& a, int ntiles, hipStream_t s)
-{
-    dim3 grid(ntiles), block(64 * N);
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage<N, 0, REALH>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((rk4_stage<N, 1, REALH>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((rk4_stage<N, 2, REALH>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((rk4_stage<N, 3, REALH>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((rk4_stage<N, 4, REALH>), grid, block, 0, s, a); break;
-    }
-}
-
-template <bool REALH>
-void launch_stage_r(int N, int stage, const StageArgs& a, int ntiles, hipStream_t s)
-{
-    switch (N) {
-    case 1: launch_stage_n<1, REALH>(stage, a, ntiles, s); break;
-    case 2: launch_stage_n<2, REALH>(stage, a, ntiles, s); break;
-    case 3: launch_stage_n<3, REALH>(stage, a, ntiles, s); break;
-    case 4: launch_stage_n<4, REALH>(stage, a, ntiles, s); break;
-    case 5: launch_stage_n<5, REALH>(stage, a, ntiles, s); break;
-    case 6: launch_stage_n<6, REALH>(stage, a, ntiles, s); break;
-    case 7: launch_stage_n<7, REALH>(stage, a, ntiles, s); break;
-    case 8: launch_stage_n<8, REALH>(stage, a, ntiles, s); break;
-    default: raise(HEOM_E_UNSUPPORTED, "states > 8 not supported by this build");
-    }
-}
-
-void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, double c_out,
-                  double c_acc)
-{
-    StageArgs a = stage_args(h);
     a.yin = yin;
     a.y0 = h->y.p;
-    a.acc = h->acc.p;
+    a.pbuf = h->pbuf.p;
     a.yout = yout;
-    a.c_out = c_out;
-    a.c_acc = c_acc;
-    if (h->real_h)
-        launch_stage_r<true>(h->N, stage, a, h->ntiles, h->stream);
-    else
-        launch_stage_r<false>(h->N, stage, a, h->ntiles, h->stream);
+    a.nb = h->nb.p;
+    a.pfss = h->pfss.p;
+    a.mp = h->Mp;
+    a.ntiles = h->ntiles;
+    a.c1 = c1;
+    ElementArgs el{h->hs.p, h->gam.p, h->terms.p};
+    heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
     HIPC(hipGetLastError());
     h->timing.rhs_launches += 1;
 }
 
+// classical RK4 in the 12-pass schedule of heom_device.hpp
 void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 {
     for (int64_t s = 0; s < nsteps; ++s) {
-        launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt, dt / 6.0);
-        launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt, dt / 3.0);
-        launch_stage(h, 3, h->yb.p, h->ya.p, dt, dt / 3.0);
-        launch_stage(h, 4, h->ya.p, h->y.p, 0.0, dt / 6.0);
+        launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt);
+        launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt);
+        launch_stage(h, 3, h->yb.p, h->ya.p, dt);
+        launch_stage(h, 4, h->ya.p, h->y.p, dt / 6.0);
     }
 }
 
 void refold_tables(heom_handle* h)
 {
-    const int64_t count = (int64_t)h->W * h->Mp;
+    const int64_t count = (int64_t)(h->W + 1) * h->Mp;
     const int threads = 256;
-    const int blocks = (int)((count + threads - 1) / threads);
-    hipLaunchKernelGGL(fold_mask, dim3(blocks), dim3(threads), 0, h->stream, h->plus_raw.p,
-                       h->plus.p, h->mask.p, h->q2u.p, count);
-    hipLaunchKernelGGL(fold_mask, dim3(blocks), dim3(threads), 0, h->stream, h->minus_raw.p,
-                       h->minus.p, h->mask.p, h->q2u.p, count);
+    const unsigned blocks = (unsigned)((count + threads - 1) / threads);
+    hipLaunchKernelGGL(fold_mask, dim3(blocks), dim3(threads), 0, h->stream, h->nb_raw.p, h->nb.p, h->mask.p,
+                       h->q2u.p, count, h->zero_slot);
     HIPC(hipGetLastError());
 }
 
@@ -587,52 +300,59 @@ void set_hamiltonian(heom_handle* h, const double* H)
     }
     if (h->hs.p == nullptr)
         h->hs.alloc(N * N);
-    HIPC(hipMemcpyAsync(h->hs.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice,
-                        h->stream));
+    HIPC(hipMemcpyAsync(h->hs.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->real_h = real;
 }
 
-// Per-element coefficient tables of the neighbour and "same" terms.  For element
+int count_bath(const heom_handle* h, int s, int b)
+{
+    int c = 0;
+    for (int t = 0; t < h->BM; ++t)
+        if (h->coupling[s * h->BM + t] == b)
+            ++c;
+    return c;
+}
+
+// Neighbour terms per matrix element and the "same"-term constants.  For element
 // (i, j) and mode m of bath b = m / K, heom_ode.cl adds, for every coupling-list entry
 // of state i equal to b, +i sqrt((n_m+1)|c_m|/hbar) sigma^{n+m} and
 // (+i c_m/hbar + [k=0] Phi_b) sqrt(n_m hbar/|c_m|) sigma^{n-m}; for every entry of
-// state j equal to b the same with -i (heom_ode.cl:184-266, 280-405).
+// state j equal to b the same with -i (heom_ode.cl:184-266, 280-405); the same term
+// subtracts Delta_b for the baths of i and of j when i != j (heom_ode.cl:138-168).
+// With c_i, c_j the number of entries equal to b in rows i and j, one Term per mode
+// carries plus_c = (c_i - c_j) sqrt(|c_m|/hbar) and
+// minus_c = (i c_m/hbar (c_i - c_j) + [k=0] Phi_b (c_i + c_j)) sqrt(hbar/|c_m|).
 void build_coefficients(heom_handle* h)
 {
-    const int N = h->N, W = h->W, K = h->K, BM = h->BM, B = h->B;
-    auto count = [&](int s, int b) {
-        int c = 0;
-        for (int t = 0; t < BM; ++t)
-            if (h->coupling[s * BM + t] == b)
-                ++c;
-        return c;
-    };
-    std::vector<int32_t> rowm_cnt(N, 0), rowm(N * W, 0), colm_cnt(N * N, 0), colm(N * N * W, 0);
-    std::vector<double> pm(N * N * W, 0.0);
-    std::vector<double2> beta(N * N * W, make_double2(0.0, 0.0)), gam(N * N, make_double2(0.0, 0.0));
+    const int N = h->N, W = h->W, K = h->K, BM = h->BM;
+    std::vector<std::vector<Term>> per(N * N);
+    std::vector<double2> gam(N * N, make_double2(0.0, 0.0));
+    int most = 0;
     for (int i = 0; i < N; ++i) {
-        for (int m = 0; m < W; ++m)
-            if (count(i, m / K) > 0)
-                rowm[i * W + rowm_cnt[i]++] = m;
         for (int j = 0; j < N; ++j) {
             const int e = i * N + j;
             for (int m = 0; m < W; ++m) {
                 const int b = m / K, k = m % K;
-                const int ci = count(i, b), cj = count(j, b);
-                if (ci == 0 && cj > 0)
-                    colm[e * W + colm_cnt[e]++] = m;
-                pm[e * W + m] = (double)(ci - cj);
+                const int ci = count_bath(h, i, b), cj = count_bath(h, j, b);
                 const double cr = h->cbk[2 * m], cim = h->cbk[2 * m + 1];
-                // i c_m / hbar (ci - cj)
-                double br = -cim / kHbar * (ci - cj);
-                double bi = cr / kHbar * (ci - cj);
+                const double a = std::hypot(cr, cim);
+                if (a == 0.0)
+                    continue;  // a mode with c_m = 0 contributes nothing (c sqrt(1/|c|) -> 0)
+                const double cp = std::sqrt(a / kHbar), cs = std::sqrt(kHbar / a);
+                Term t{};
+                t.mode = m;
+                t.plus_c = (ci - cj) * cp;
+                double br = -cim / kHbar * (ci - cj), bi = cr / kHbar * (ci - cj);
                 if (k == 0) {
                     br += h->pf_circ[2 * b] * (ci + cj);
                     bi += h->pf_circ[2 * b + 1] * (ci + cj);
                 }
-                beta[e * W + m] = make_double2(br, bi);
+                t.minus_c = make_double2(br * cs, bi * cs);
+                if (t.plus_c != 0.0 || t.minus_c.x != 0.0 || t.minus_c.y != 0.0)
+                    per[e].push_back(t);
             }
+            most = std::max(most, (int)per[e].size());
             if (i != j) {
                 double gr = 0.0, gi = 0.0;
                 for (int t = 0; t < BM; ++t) {
@@ -651,25 +371,22 @@ void build_coefficients(heom_handle* h)
             }
         }
     }
-    (void)B;
-    // sqrt factors exactly as heom_ode.cl:211,249 and :321,383 evaluate them
-    std::vector<double> pcoef(W * h->nv, 0.0), scoef(W * h->nv, 0.0);
-    for (int m = 0; m < W; ++m) {
-        const double a = std::hypot(h->cbk[2 * m], h->cbk[2 * m + 1]);
-        for (int v = 0; v < h->nv; ++v) {
-            pcoef[m * h->nv + v] = std::sqrt((v + 1.0) * a / kHbar);
-            scoef[m * h->nv + v] = v > 0 ? std::sqrt(v / a * kHbar) : 0.0;
+    if (!stage_supported(N, most))
+        raise(HEOM_E_UNSUPPORTED, "more than 8 neighbour terms per matrix element");
+    h->term_slots = stage_term_slots(most);
+    // pad every element's list; padding terms read table column W, which always
+    // holds the zero slot
+    std::vector<Term> flat((size_t)N * N * h->term_slots);
+    for (int e = 0; e < N * N; ++e)
+        for (int k = 0; k < h->term_slots; ++k) {
+            Term t{};
+            t.mode = W;
+            t.plus_c = 0.0;
+            t.minus_c = make_double2(0.0, 0.0);
+            flat[(size_t)e * h->term_slots + k] = k < (int)per[e].size() ? per[e][k] : t;
         }
-    }
-    upload(h->rowm_cnt, rowm_cnt, h->stream);
-    upload(h->rowm, rowm, h->stream);
-    upload(h->colm_cnt, colm_cnt, h->stream);
-    upload(h->colm, colm, h->stream);
-    upload(h->pm, pm, h->stream);
-    upload(h->beta, beta, h->stream);
+    upload(h->terms, flat, h->stream);
     upload(h->gam, gam, h->stream);
-    upload(h->pcoef, pcoef, h->stream);
-    upload(h->scoef, scoef, h->stream);
 }
 
 void validate(const heom_desc* d)
@@ -684,12 +401,14 @@ void validate(const heom_desc* d)
         raise(HEOM_E_INVALID, "matsubaras, baths and baths_max_per_state must be > 0");
     if (d->width != d->baths * d->matsubaras)
         raise(HEOM_E_INVALID, "width must equal baths * matsubaras");
-    if (d->matrices < 1 || d->matrices > (int64_t)INT32_MAX - kTile)
-        raise(HEOM_E_INVALID, "matrices out of range");
+    if (d->matrices < 1)
+        raise(HEOM_E_INVALID, "matrices must be > 0");
+    if (d->matrices > kMaxAdos - 2 * kTile)
+        raise(HEOM_E_UNSUPPORTED, "more than 2^26 - 129 ADOs");
     if (d->precision != HEOM_PRECISION_FP64)
         raise(HEOM_E_UNSUPPORTED, "only HEOM_PRECISION_FP64 is supported by this build");
-    if (!d->coupling || !d->hamiltonian || !d->ado_tuples || !d->ado_plus || !d->ado_minus ||
-        !d->pf_same_sum || !d->pf_same || !d->pf_circ || !d->cbk)
+    if (!d->coupling || !d->hamiltonian || !d->ado_tuples || !d->ado_plus || !d->ado_minus || !d->pf_same_sum ||
+        !d->pf_same || !d->pf_circ || !d->cbk)
         raise(HEOM_E_INVALID, "desc has a NULL array");
     for (int t = 0; t < d->states * d->baths_max_per_state; ++t)
         if (d->coupling[t] >= d->baths)
@@ -719,6 +438,9 @@ void need(heom_handle* h)
     if (!h)
         raise(HEOM_E_INVALID, "handle is NULL");
 }
+
+unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
 }  // namespace
 
 extern "C" {
@@ -735,7 +457,7 @@ int heom_create(const heom_desc* d, heom_handle** out)
     }
     *out = nullptr;
     heom_handle* h = nullptr;
-    int rc = guard([&] {
+    const int rc = guard([&] {
         validate(d);
         h = new heom_handle();
         HIPC(hipGetDevice(&h->device));
@@ -751,8 +473,9 @@ int heom_create(const heom_desc* d, heom_handle** out)
         h->W = W;
         h->nel = N * N;
         h->M = M;
-        h->Mp = (M + kTile - 1) / kTile * kTile;
+        h->Mp = (M + 1 + kTile - 1) / kTile * kTile;  // at least one zero slot
         h->ntiles = (int)(h->Mp / kTile);
+        h->zero_slot = (int32_t)M;
         h->coupling.assign(d->coupling, d->coupling + N * h->BM);
         h->cbk.assign(d->cbk, d->cbk + 2 * h->B * h->K);
         h->pf_same.assign(d->pf_same, d->pf_same + 2 * h->B);
@@ -760,42 +483,28 @@ int heom_create(const heom_desc* d, heom_handle** out)
 
         // lexicographic ADO order (mode 0 most significant)
         const int32_t* T = d->ado_tuples;
-        int maxv = 0;
-        for (int64_t t = 0; t < M * W; ++t) {
-            if (T[t] < 0 || T[t] > 255)
-                raise(HEOM_E_UNSUPPORTED, "tuple entries must be in [0, 255]");
-            maxv = std::max(maxv, (int)T[t]);
-        }
-        h->nv = maxv + 2;
+        for (int64_t t = 0; t < M * W; ++t)
+            if (T[t] < 0 || T[t] > kMaxTupleValue)
+                raise(HEOM_E_UNSUPPORTED, "tuple entries must be in [0, 62]");
         std::vector<int32_t> order(M);
         std::iota(order.begin(), order.end(), 0);
         std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W,
-                                                T + (int64_t)b * W, T + (int64_t)b * W + W);
+            return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W, T + (int64_t)b * W,
+                                                T + (int64_t)b * W + W);
         });
         const int64_t Mp = h->Mp;
-        h->q2u_host.assign(Mp, -1);
-        std::vector<int32_t> u2q(M);
+        std::vector<int32_t> q2u(Mp, -1), u2q(M);
         for (int64_t q = 0; q < M; ++q) {
-            h->q2u_host[q] = order[q];
+            q2u[q] = order[q];
             u2q[order[q]] = (int32_t)q;
         }
-        h->q_top = -1;
-        for (int64_t u = 0; u < M; ++u) {
-            bool zero = true;
-            for (int m = 0; m < W; ++m)
-                zero = zero && T[u * W + m] == 0;
-            if (zero) {
-                if (u != 0)
-                    raise(HEOM_E_INVALID, "the all-zero tuple (top ADO) must be uid 0");
-                h->q_top = u2q[u];
-            }
-        }
-        if (h->q_top < 0)
-            raise(HEOM_E_INVALID, "no all-zero tuple (top ADO) in ado_tuples");
+        for (int m = 0; m < W; ++m)
+            if (T[m] != 0)
+                raise(HEOM_E_INVALID, "uid 0 must be the top ADO (all-zero tuple)");
+        h->q_top = u2q[0];
 
-        std::vector<int32_t> plus(W * Mp, -1), minus(W * Mp, -1);
-        std::vector<uint8_t> tup(W * Mp, 0);
+        const int32_t zero = h->zero_slot;
+        std::vector<int2> nb((W + 1) * Mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
         std::vector<double2> pfss(Mp, make_double2(0.0, 0.0));
         for (int64_t q = 0; q < M; ++q) {
             const int64_t u = order[q];
@@ -804,33 +513,26 @@ int heom_create(const heom_desc* d, heom_handle** out)
                 const int32_t mn = d->ado_minus[u * W + m];
                 if (p >= M || mn >= M)
                     raise(HEOM_E_INVALID, "plus/minus entry out of range");
-                plus[m * Mp + q] = p >= 0 ? u2q[p] : -1;
-                minus[m * Mp + q] = (mn >= 0 && T[u * W + m] > 0) ? u2q[mn] : -1;
-                tup[m * Mp + q] = (uint8_t)T[u * W + m];
+                const int32_t pq = p >= 0 ? u2q[p] : zero;
+                const int32_t mq = (mn >= 0 && T[u * W + m] > 0) ? u2q[mn] : zero;
+                nb[m * Mp + q] = make_int2((int32_t)(((uint32_t)pq << 6) | (uint32_t)T[u * W + m]), mq);
             }
             pfss[q] = make_double2(d->pf_same_sum[2 * u], d->pf_same_sum[2 * u + 1]);
         }
-        upload(h->plus_raw, plus, h->stream);
-        upload(h->minus_raw, minus, h->stream);
-        upload(h->plus, plus, h->stream);
-        upload(h->minus, minus, h->stream);
-        upload(h->tup, tup, h->stream);
+        upload(h->nb_raw, nb, h->stream);
+        upload(h->nb, nb, h->stream);
         upload(h->pfss, pfss, h->stream);
-        upload(h->q2u, h->q2u_host, h->stream);
+        upload(h->q2u, q2u, h->stream);
         h->mask.alloc(M);
         HIPC(hipMemsetAsync(h->mask.p, 1, M, h->stream));
         h->norms.alloc(M);
         h->mx.alloc(N * N);
         h->small.alloc(N * N);
         const size_t plane = (size_t)N * N * Mp;
-        h->y.alloc(plane);
-        h->acc.alloc(plane);
-        h->ya.alloc(plane);
-        h->yb.alloc(plane);
-        HIPC(hipMemsetAsync(h->y.p, 0, h->y.bytes(), h->stream));
-        HIPC(hipMemsetAsync(h->acc.p, 0, h->acc.bytes(), h->stream));
-        HIPC(hipMemsetAsync(h->ya.p, 0, h->ya.bytes(), h->stream));
-        HIPC(hipMemsetAsync(h->yb.p, 0, h->yb.bytes(), h->stream));
+        for (auto* b : {&h->y, &h->pbuf, &h->ya, &h->yb}) {
+            b->alloc(plane);
+            HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+        }
         build_coefficients(h);
         set_hamiltonian(h, d->hamiltonian);
         HIPC(hipStreamSynchronize(h->stream));
@@ -895,9 +597,8 @@ int heom_set_state(heom_handle* h, const double* state)
             raise(HEOM_E_INVALID, "state is NULL");
         const size_t bytes = (size_t)h->M * h->nel * sizeof(double2);
         HIPC(hipMemcpyAsync(h->ya.p, state, bytes, hipMemcpyHostToDevice, h->stream));
-        const int threads = 256;
-        hipLaunchKernelGGL(aos_to_soa, dim3((unsigned)((h->Mp + threads - 1) / threads)), dim3(threads), 0,
-                           h->stream, h->ya.p, h->y.p, h->q2u.p, h->Mp, h->nel);
+        hipLaunchKernelGGL(aos_to_soa, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->ya.p, h->y.p,
+                           h->q2u.p, h->Mp, h->nel);
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
     });
@@ -910,9 +611,18 @@ int heom_set_top(heom_handle* h, const double* top)
         if (!top)
             raise(HEOM_E_INVALID, "top is NULL");
         HIPC(hipMemcpyAsync(h->small.p, top, h->nel * sizeof(double2), hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(copy_ado_in, dim3(1), dim3(((h->nel + 63) / 64) * 64), 0, h->stream, h->small.p,
-                           h->y.p, h->q_top, h->Mp, h->nel);
+        hipLaunchKernelGGL(copy_ado_in, dim3(1), dim3(((h->nel + 63) / 64) * 64), 0, h->stream, h->small.p, h->y.p,
+                           h->q_top, h->Mp, h->nel);
         HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(h->stream));
+    });
+}
+
+int heom_clear_state(heom_handle* h)
+{
+    return guard([&] {
+        need(h);
+        HIPC(hipMemsetAsync(h->y.p, 0, h->y.bytes(), h->stream));
         HIPC(hipStreamSynchronize(h->stream));
     });
 }
@@ -923,12 +633,11 @@ int heom_get_state(heom_handle* h, double* state)
         need(h);
         if (!state)
             raise(HEOM_E_INVALID, "state is NULL");
-        const int threads = 256;
-        hipLaunchKernelGGL(soa_to_aos, dim3((unsigned)((h->Mp + threads - 1) / threads)), dim3(threads), 0,
-                           h->stream, h->y.p, h->yb.p, h->q2u.p, h->Mp, h->nel);
+        hipLaunchKernelGGL(soa_to_aos, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->y.p, h->yb.p,
+                           h->q2u.p, h->Mp, h->nel);
         HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(state, h->yb.p, (size_t)h->M * h->nel * sizeof(double2),
-                            hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(state, h->yb.p, (size_t)h->M * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
+                            h->stream));
         HIPC(hipStreamSynchronize(h->stream));
     });
 }
@@ -953,10 +662,8 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
         need(h);
         if (!norms)
             raise(HEOM_E_INVALID, "norms is NULL");
-        const int threads = 256;
-        hipLaunchKernelGGL(hierarchy_norm_kernel, dim3((unsigned)((h->Mp + threads - 1) / threads)),
-                           dim3(threads), 0, h->stream, h->y.p, h->q2u.p, h->Mp, h->nel, threshold,
-                           h->norms.p, h->mask.p);
+        hipLaunchKernelGGL(hierarchy_norm_kernel, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->y.p,
+                           h->q2u.p, h->Mp, h->nel, threshold, h->norms.p, h->mask.p);
         HIPC(hipGetLastError());
         if (threshold >= 0.0)
             refold_tables(h);
@@ -971,10 +678,9 @@ int heom_rhs(heom_handle* h, double* out)
         need(h);
         if (!out)
             raise(HEOM_E_INVALID, "out is NULL");
-        launch_stage(h, 0, h->y.p, h->ya.p, 0.0, 0.0);
-        const int threads = 256;
-        hipLaunchKernelGGL(soa_to_aos, dim3((unsigned)((h->Mp + threads - 1) / threads)), dim3(threads), 0,
-                           h->stream, h->ya.p, h->yb.p, h->q2u.p, h->Mp, h->nel);
+        launch_stage(h, 0, h->y.p, h->ya.p, 0.0);
+        hipLaunchKernelGGL(soa_to_aos, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->ya.p, h->yb.p,
+                           h->q2u.p, h->Mp, h->nel);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(out, h->yb.p, (size_t)h->M * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
                             h->stream));
@@ -1022,8 +728,8 @@ int heom_propagate(heom_handle* h, double dt, int64_t nsteps, int64_t observe_st
             rk4_steps(h, dt, n);
             done += n;
             if (n == observe_steps) {
-                hipLaunchKernelGGL(copy_ado_out, dim3(1), dim3(tb), 0, h->stream, h->y.p,
-                                   rec.p + row * h->nel, h->q_top, h->Mp, h->nel);
+                hipLaunchKernelGGL(copy_ado_out, dim3(1), dim3(tb), 0, h->stream, h->y.p, rec.p + row * h->nel,
+                                   h->q_top, h->Mp, h->nel);
                 ++row;
             }
         }
@@ -1035,7 +741,6 @@ int heom_propagate(heom_handle* h, double dt, int64_t nsteps, int64_t observe_st
                             h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         resolve_timing(h);
-        rec.free();
     });
 }
 
@@ -1046,17 +751,16 @@ int heom_mmult_left(heom_handle* h, const double* mx)
         if (!mx)
             raise(HEOM_E_INVALID, "mx is NULL");
         HIPC(hipMemcpyAsync(h->mx.p, mx, h->nel * sizeof(double2), hipMemcpyHostToDevice, h->stream));
-        dim3 grid(h->ntiles), block(64 * h->N);
+        const dim3 grid(h->ntiles), block(64 * h->N);
         switch (h->N) {
-        case 1: hipLaunchKernelGGL(mmult_left_kernel<1>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 2: hipLaunchKernelGGL(mmult_left_kernel<2>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 3: hipLaunchKernelGGL(mmult_left_kernel<3>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 4: hipLaunchKernelGGL(mmult_left_kernel<4>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 5: hipLaunchKernelGGL(mmult_left_kernel<5>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 6: hipLaunchKernelGGL(mmult_left_kernel<6>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 7: hipLaunchKernelGGL(mmult_left_kernel<7>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        case 8: hipLaunchKernelGGL(mmult_left_kernel<8>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp, h->ntiles); break;
-        default: raise(HEOM_E_UNSUPPORTED, "states > 8");
+        case 1: hipLaunchKernelGGL(mmult_left_kernel<1>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 2: hipLaunchKernelGGL(mmult_left_kernel<2>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 3: hipLaunchKernelGGL(mmult_left_kernel<3>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 4: hipLaunchKernelGGL(mmult_left_kernel<4>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 5: hipLaunchKernelGGL(mmult_left_kernel<5>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 6: hipLaunchKernelGGL(mmult_left_kernel<6>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        case 7: hipLaunchKernelGGL(mmult_left_kernel<7>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        default: hipLaunchKernelGGL(mmult_left_kernel<8>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
         }
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
@@ -1090,12 +794,9 @@ int heom_device_bytes(heom_handle* h, int64_t* bytes)
         if (!bytes)
             raise(HEOM_E_INVALID, "bytes is NULL");
         int64_t b = 0;
-        for (auto* x : {&h->y, &h->acc, &h->ya, &h->yb, &h->pfss, &h->hs, &h->gam, &h->beta, &h->mx, &h->small})
+        for (auto* x : {&h->y, &h->pbuf, &h->ya, &h->yb, &h->pfss, &h->hs, &h->gam, &h->mx, &h->small})
             b += x->bytes();
-        for (auto* x : {&h->plus_raw, &h->minus_raw, &h->plus, &h->minus, &h->q2u, &h->rowm_cnt, &h->rowm,
-                        &h->colm_cnt, &h->colm})
-            b += x->bytes();
-        b += h->tup.bytes() + h->mask.bytes() + h->pm.bytes() + h->pcoef.bytes() + h->scoef.bytes() +
+        b += h->nb_raw.bytes() + h->nb.bytes() + h->q2u.bytes() + h->mask.bytes() + h->terms.bytes() +
              h->norms.bytes();
         *bytes = b;
     });

@@ -1,0 +1,174 @@
+// stage_kernel.hip -- the HEOM RHS with the RK4 stage combination fused into its
+// epilogue (dm-heom/cl/heom/heom_ode.cl:24-451 plus rk_weighted_add.cl:19-106).
+//
+// Work split: a workgroup owns a tile of 64 consecutive ADOs (lane = ADO); each
+// wavefront computes a block of CW = 4 columns j of one matrix row i for those 64 ADOs
+// (N * ceil(N/4) wavefronts per workgroup, 16 for N = 8).  Every global access of a
+// wavefront is therefore one element plane at 64 consecutive (or, for neighbours,
+// nearly consecutive) ADOs, and everything that depends only on the element --
+// H / hbar, the "same" constant, the neighbour-term list -- is wave-uniform and comes
+// through scalar loads.  Per wavefront:
+//   * own elements sigma_ij (j in the block) -> registers and the LDS tile of the 64
+//     ADOs x N^2 elements; barrier;
+//   * row i of sigma from the tile (shared by the block's columns), then per column j:
+//     commutator (i/hbar)(sigma H - H sigma)_ij with column j from the tile, the "same"
+//     term (S_n + gamma_ij) sigma_ij, and the element's neighbour terms: for each
+//     (mode m, plus constant, minus constant) triple built on the host from the
+//     bath-coupling list, the plus / minus neighbour indices and n_m come from one
+//     8-byte entry of the mode-major table and two gathers from the element plane give
+//     f += i c_p sqrt(n_m+1) sigma^{n+e_m} + c_m sqrt(n_m) sigma^{n-e_m}
+//     (missing or masked edges read the zero slot);
+//   * the RK4 epilogue of heom_device.hpp.
+// Any coupling (several baths per state, shared baths, uncoupled states) runs through
+// the same code; the term loop is unrolled so a column's table loads and gathers are
+// all in flight together.
+#include "heom_device.hpp"
+
+namespace heom_hip {
+
+namespace {
+
+template <int N, int NT, int STAGE, bool REALH>
+__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, const double2* __restrict__ hs,
+                                                                  const double2* __restrict__ gam,
+                                                                  const Term* __restrict__ terms)
+{
+    constexpr int CW = kStageColumns;
+    constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
+    __shared__ double2 sig[N * N * kTile];
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = w / NJW;
+    const int jb = (w % NJW) * CW;
+    const int tile = xcd_tile(blockIdx.x, a.ntiles);
+    const int64_t mp = a.mp;
+    const uint32_t q = (uint32_t)tile * kTile + lane;
+
+    double2 s[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const int j = jb + c;
+        if (j < N) {
+            s[c] = a.yin[(int64_t)(i * N + j) * mp + q];
+            sig[(i * N + j) * kTile + lane] = s[c];
+        } else {
+            s[c] = make_double2(0.0, 0.0);
+        }
+    }
+    const double2 S = a.pfss[q];
+    __syncthreads();
+
+    double2 srow[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        srow[k] = sig[(i * N + k) * kTile + lane];
+    const double sq_lane = sqrt((double)lane);  // sqrt(n), n = 0..63, one value per lane
+
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const int j = jb + c;
+        if (j >= N)
+            break;
+        const int e = i * N + j;
+        const double2* plane = a.yin + (int64_t)e * mp;
+
+        // neighbour terms of this element: tables and gathers first
+        int2 nbv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k)
+            nbv[k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
+        double2 xp[NT], xm[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            xp[k] = plane[(uint32_t)nbv[k].x >> 6];
+            xm[k] = plane[(uint32_t)nbv[k].y];
+        }
+
+        // commutator c = sum_k sigma_ik H_kj - H_ik sigma_kj
+        double2 cm = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double2 skj = sig[(k * N + j) * kTile + lane];
+            const double2 hkj = hs[k * N + j];
+            const double2 hik = hs[i * N + k];
+            if (REALH) {
+                cm.x = fma(srow[k].x, hkj.x, fma(-hik.x, skj.x, cm.x));
+                cm.y = fma(srow[k].y, hkj.x, fma(-hik.x, skj.y, cm.y));
+            } else {
+                cm = cfma(srow[k], hkj, cm);
+                cm = cfma(make_double2(-hik.x, -hik.y), skj, cm);
+            }
+        }
+        const double2 g = gam[e];
+        double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), s[c], make_double2(-cm.y, cm.x));
+
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const Term t = terms[e * NT + k];
+            const int v = nbv[k].x & 63;
+            const double cp = t.plus_c * lane_value(sq_lane, v + 1);
+            const double sm = lane_value(sq_lane, v);
+            f.x = fma(-cp, xp[k].y, f.x);
+            f.y = fma(cp, xp[k].x, f.y);
+            f = cfma(t.minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
+        }
+
+        stage_epilogue<STAGE>(a, (int64_t)e * mp + q, s[c], f);
+    }
+}
+
+template <int N, int NT, bool REALH>
+void launch_t(int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+{
+    const dim3 grid(a.ntiles), block(64 * stage_waves(N));
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((rk4_stage<N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage<N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage<N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage<N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage<N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    }
+}
+
+template <int N, bool REALH>
+void launch_n(int nt, int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+{
+    switch (nt) {
+    case 2: launch_t<N, 2, REALH>(stage, a, el, st); break;
+    case 4: launch_t<N, 4, REALH>(stage, a, el, st); break;
+    default: launch_t<N, 8, REALH>(stage, a, el, st); break;
+    }
+}
+
+template <bool REALH>
+void launch_r(int N, int nt, int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+{
+    switch (N) {
+    case 1: launch_n<1, REALH>(nt, stage, a, el, st); break;
+    case 2: launch_n<2, REALH>(nt, stage, a, el, st); break;
+    case 3: launch_n<3, REALH>(nt, stage, a, el, st); break;
+    case 4: launch_n<4, REALH>(nt, stage, a, el, st); break;
+    case 5: launch_n<5, REALH>(nt, stage, a, el, st); break;
+    case 6: launch_n<6, REALH>(nt, stage, a, el, st); break;
+    case 7: launch_n<7, REALH>(nt, stage, a, el, st); break;
+    default: launch_n<8, REALH>(nt, stage, a, el, st); break;
+    }
+}
+
+}  // namespace
+
+int stage_term_slots(int nterms) { return nterms <= 2 ? 2 : nterms <= 4 ? 4 : 8; }
+
+bool stage_supported(int N, int nterms) { return N >= 1 && N <= kMaxStates && nterms >= 0 && nterms <= 8; }
+
+void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a, const ElementArgs& el,
+                  hipStream_t s)
+{
+    if (real_h)
+        launch_r<true>(N, nterm_slots, stage, a, el, s);
+    else
+        launch_r<false>(N, nterm_slots, stage, a, el, s);
+}
+
+}  // namespace heom_hip

@@ -17,7 +17,10 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(ROOT, "lib", "libheom_hip.so")
+HOST_LIB_PATH = os.path.join(ROOT, "lib", "libheom_host.so")
 HEADER = os.path.join(os.path.dirname(ROOT), "include", "heom_hip.h")
+HOST_HEADER = os.path.join(os.path.dirname(ROOT), "include", "heom_host.h")
+CONFIG_DIR = os.path.join(ROOT, "configs")
 
 HEOM_OK = 0
 HEOM_E_INVALID = -1
@@ -30,6 +33,7 @@ HEOM_PRECISION_FP64 = 0
 EXPORTED = [
     "heom_create", "heom_destroy", "heom_last_error", "heom_abi_version",
     "heom_set_hamiltonian", "heom_set_mask", "heom_get_mask", "heom_set_state", "heom_set_top",
+    "heom_clear_state",
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes",
@@ -86,6 +90,7 @@ def load():
             "heom_get_mask": [vp, vp],
             "heom_set_state": [vp, vp],
             "heom_set_top": [vp, vp],
+            "heom_clear_state": [vp],
             "heom_get_state": [vp, vp],
             "heom_get_top": [vp, vp],
             "heom_hierarchy_norm": [vp, dbl, vp],
@@ -124,6 +129,19 @@ def _c128_flat(m) -> np.ndarray:
 
 class Handle:
     """One hierarchy on the GPU (owns all its device memory)."""
+
+    @classmethod
+    def from_desc(cls, desc: Desc, keepalive=None) -> "Handle":
+        """Create from a filled descriptor (e.g. HostInstance.desc())."""
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _check(load().heom_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+        self.states = desc.states
+        self.matrices = desc.matrices
+        self.width = desc.width
+        self._keep = keepalive
+        return self
 
     def __init__(self, *, states, matsubaras, baths, coupling, hamiltonian, tuples, plus, minus,
                  pf_same_sum, pf_same, pf_circ, cbk, precision=HEOM_PRECISION_FP64):
@@ -203,6 +221,9 @@ class Handle:
         t = _c128_flat(top)
         _check(load().heom_set_top(self._h, t.ctypes.data))
 
+    def clear_state(self) -> None:
+        _check(load().heom_clear_state(self._h))
+
     def set_hamiltonian(self, H) -> None:
         h = _c128_flat(H) if np.iscomplexobj(H) else _f64(H)
         _check(load().heom_set_hamiltonian(self._h, h.ctypes.data))
@@ -272,3 +293,168 @@ class Handle:
         b = ctypes.c_int64()
         _check(load().heom_rk4_step_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+
+# --------------------------------------------------------------------------------------
+# host side: the config-file instance API (libheom_host.so, include/heom_host.h)
+# --------------------------------------------------------------------------------------
+
+TASK_POPULATION_DYNAMICS = 0
+TASK_LINEAR_ABSORPTION = 1
+
+HOST_EXPORTED = [
+    "heom_host_open", "heom_host_close", "heom_host_last_error", "heom_host_get_info", "heom_host_desc",
+    "heom_host_top", "heom_host_cbk_sqrt", "heom_host_mask", "heom_host_dipoles", "heom_host_partitions",
+]
+
+
+class HostInfo(ctypes.Structure):
+    _fields_ = [("sites", ctypes.c_int64), ("states", ctypes.c_int64), ("baths", ctypes.c_int64),
+                ("matsubaras", ctypes.c_int64), ("width", ctypes.c_int64), ("depth", ctypes.c_int64),
+                ("matrices", ctypes.c_int64), ("first_non_plus", ctypes.c_int64),
+                ("step_size", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("observe_steps", ctypes.c_int64), ("tensor_components", ctypes.c_int64)]
+
+
+class ConfigError(ValueError):
+    """A config file the reference would reject (heom::config_error)."""
+
+
+_host = None
+
+
+def load_host():
+    """Load libheom_host.so (raises if it has not been built)."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise HeomError(HEOM_E_UNSUPPORTED, f"libheom_host.so not built: {HOST_LIB_PATH}")
+        L = ctypes.CDLL(HOST_LIB_PATH)
+        vp = ctypes.c_void_p
+        sig = {
+            "heom_host_open": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(vp)], ctypes.c_int),
+            "heom_host_close": ([vp], ctypes.c_int),
+            "heom_host_get_info": ([vp, ctypes.POINTER(HostInfo)], ctypes.c_int),
+            "heom_host_desc": ([vp, ctypes.POINTER(Desc)], ctypes.c_int),
+            "heom_host_top": ([vp, vp], ctypes.c_int),
+            "heom_host_cbk_sqrt": ([vp, vp], ctypes.c_int),
+            "heom_host_mask": ([vp, vp], ctypes.c_int),
+            "heom_host_dipoles": ([vp, ctypes.c_int64, vp, vp], ctypes.c_int),
+            "heom_host_partitions": ([ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64], ctypes.c_int64),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        L.heom_host_last_error.argtypes = []
+        L.heom_host_last_error.restype = ctypes.c_char_p
+        _host = L
+    return _host
+
+
+def _host_check(rc: int) -> None:
+    if rc != 0:
+        msg = load_host().heom_host_last_error().decode()
+        if rc == -5:
+            raise ConfigError(msg)
+        raise HeomError(rc, msg)
+
+
+def partitions(n: int, s: int) -> np.ndarray:
+    """Integer partitions of n into at most s parts, in the reference's order."""
+    L = load_host()
+    cnt = L.heom_host_partitions(n, s, None, 0)
+    if cnt < 0:
+        _host_check(int(cnt))
+    out = np.zeros((max(cnt, 1), s), dtype=np.int32)
+    L.heom_host_partitions(n, s, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
+class HostInstance:
+    """A reference config file turned into kernel inputs by the C++ host code:
+    config -> hierarchy_graph -> instance (population dynamics: identity mapping and
+    rho_init on the top ADO; linear absorption: a ground state and |g><g|)."""
+
+    def __init__(self, config_path: str, task: int = TASK_POPULATION_DYNAMICS, depth: int = 0):
+        h = ctypes.c_void_p()
+        _host_check(load_host().heom_host_open(str(config_path).encode(), int(task), int(depth), ctypes.byref(h)))
+        self._p = h
+        self.task = task
+        info = HostInfo()
+        _host_check(load_host().heom_host_get_info(h, ctypes.byref(info)))
+        self.info = info
+        self.states = int(info.states)
+        self.matrices = int(info.matrices)
+        self.width = int(info.width)
+
+    def close(self) -> None:
+        if getattr(self, "_p", None):
+            load_host().heom_host_close(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def desc(self) -> Desc:
+        d = Desc()
+        _host_check(load_host().heom_host_desc(self._p, ctypes.byref(d)))
+        return d
+
+    def arrays(self) -> dict:
+        """Copies of the kernel-input arrays (reference layout)."""
+        d = self.desc()
+        N, M, W, B, K = d.states, d.matrices, d.width, d.baths, d.matsubaras
+
+        def arr(ptr, n, ctype, dtype):
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,)).astype(dtype).copy()
+
+        i32, f64 = ctypes.c_int32, ctypes.c_double
+        out = dict(
+            coupling=arr(d.coupling, N * d.baths_max_per_state, i32, np.int32),
+            hamiltonian=arr(d.hamiltonian, 2 * N * N, f64, np.float64),
+            tuples=arr(d.ado_tuples, M * W, i32, np.int32),
+            plus=arr(d.ado_plus, M * W, i32, np.int32),
+            minus=arr(d.ado_minus, M * W, i32, np.int32),
+            pf_same_sum=arr(d.pf_same_sum, 2 * M, f64, np.float64),
+            pf_same=arr(d.pf_same, 2 * B, f64, np.float64),
+            pf_circ=arr(d.pf_circ, 2 * B, f64, np.float64),
+            cbk=arr(d.cbk, 2 * B * K, f64, np.float64),
+        )
+        cs = np.zeros(B * K)
+        _host_check(load_host().heom_host_cbk_sqrt(self._p, cs.ctypes.data))
+        out["cbk_sqrt"] = cs
+        return out
+
+    def top(self) -> np.ndarray:
+        t = np.zeros(2 * self.states * self.states)
+        _host_check(load_host().heom_host_top(self._p, t.ctypes.data))
+        t = t.reshape(self.states, self.states, 2)
+        return t[..., 0] + 1j * t[..., 1]
+
+    def mask(self) -> np.ndarray:
+        m = np.zeros(self.matrices, dtype=np.uint8)
+        _host_check(load_host().heom_host_mask(self._p, m.ctypes.data))
+        return m
+
+    def dipoles(self, t: int):
+        n = self.states
+        dm, dp = np.zeros(2 * n * n), np.zeros(2 * n * n)
+        _host_check(load_host().heom_host_dipoles(self._p, int(t), dm.ctypes.data, dp.ctypes.data))
+        dm, dp = dm.reshape(n, n, 2), dp.reshape(n, n, 2)
+        return dm[..., 0] + 1j * dm[..., 1], dp[..., 0] + 1j * dp[..., 1]
+
+    def create_handle(self) -> Handle:
+        """heom_create on the instance, with its initial top ADO loaded."""
+        h = Handle.from_desc(self.desc(), keepalive=self)
+        h.set_top(self.top())
+        return h

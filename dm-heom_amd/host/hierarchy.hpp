@@ -1,0 +1,115 @@
+// hierarchy.hpp -- hierarchy graph, sites-to-states mappings and the instance
+// (kernel inputs) of the HEOM population-dynamics / linear-absorption path.
+//
+// hierarchy_graph reproduces the reference's ADO numbering exactly
+// (dm-heom/src/heom/hierarchy_graph.cpp:30-181): layer by layer (depth 0..D); inside a
+// layer, the integer partitions of the depth into at most W parts in increasing
+// lexicographic order of their non-increasing part sequences (the order the
+// reference's partition generator emits, dm-heom/src/heom/integer_partition.cpp:41-105),
+// each padded with zeros, sorted ascending and expanded into all of its distinct
+// permutations in lexicographic order.  The ADO order is part of the reference's data
+// contract, so it is kept; the device library re-orders internally.
+//
+// instance computes the kernel inputs of dm-heom/src/heom/instance.cpp:25-69,155-262
+// (same formulas and evaluation order, std::complex<double>, the per-ADO sum in long
+// double).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "heom_hip.h"
+
+namespace heom {
+
+using uid_t = std::int32_t;
+constexpr uid_t invalid_uid = -2;  // dm-heom/src/heom/hierarchy_graph.cpp:24
+
+// exact binomial coefficient; throws std::overflow_error beyond int64
+int_t binomial(int_t n, int_t k);
+
+// partitions of n into at most s parts, increasing lexicographic order of the
+// non-increasing part sequences, each padded with zeros to s entries
+std::vector<std::vector<int>> integer_partitions(int n, int s);
+
+class hierarchy_graph {
+public:
+    hierarchy_graph(int_t width, int_t depth);
+    // reference constructor form: width = baths * matsubaras
+    hierarchy_graph(int_t baths, int_t matsubaras, int_t depth) : hierarchy_graph(baths * matsubaras, depth) {}
+
+    int_t width() const { return width_; }
+    int_t depth() const { return depth_; }
+    int_t nodes() const { return nodes_; }
+    int_t first_non_plus_uid() const { return first_non_plus_; }
+    const std::vector<int_t>& nodes_per_depth() const { return nodes_per_depth_; }
+    // flat [nodes * width] arrays
+    const std::vector<std::int32_t>& tuples() const { return tuples_; }
+    const std::vector<std::int32_t>& plus() const { return plus_; }
+    const std::vector<std::int32_t>& minus() const { return minus_; }
+
+private:
+    int_t width_, depth_, nodes_, first_non_plus_;
+    std::vector<int_t> nodes_per_depth_;
+    std::vector<std::int32_t> tuples_, plus_, minus_;
+};
+
+// dm-heom/include/heom/sites_to_states.hpp
+enum class sites_to_states_mode { identity, with_ground_state };
+
+int_t sites_to_states(int_t sites, sites_to_states_mode mode);
+complex_matrix state_hamiltonian(const complex_matrix& h, sites_to_states_mode mode);
+int_matrix state_baths_coupling(const int_matrix& coupling, sites_to_states_mode mode);
+
+// mask values (dm-heom/cl/heom/common.cl:14-19)
+enum mask_value : std::uint8_t { mask_false = 0, mask_true = 1, mask_protected = 2 };
+// create_hierarchy_mask for strategies "none" and "single_excitation"
+// (dm-heom/src/heom/hierarchy_mask.cpp:20-110)
+std::vector<std::uint8_t> create_hierarchy_mask(const std::string& strategy, const hierarchy_graph& graph,
+                                                int_t matsubaras, int_t first_layer);
+
+class instance {
+public:
+    instance(const config& cfg, sites_to_states_mode mode, const hierarchy_graph& graph);
+
+    int_t states() const { return states_; }
+    int_t matsubaras() const { return matsubaras_; }
+    int_t baths() const { return baths_; }
+    int_t baths_per_state() const { return baths_per_state_; }
+    int_t ado_width() const { return width_; }
+    int_t ado_depth() const { return depth_; }
+    int_t matrices() const { return matrices_; }
+    int_t first_non_plus_id() const { return first_non_plus_; }
+    sites_to_states_mode mode() const { return mode_; }
+
+    const std::vector<double>& hamiltonian() const { return hamiltonian_; }   // [N*N*2]
+    const std::vector<std::int32_t>& coupling() const { return coupling_; }   // [N*baths_per_state]
+    const std::vector<double>& pf_same_sum() const { return pf_same_sum_; }   // [2M]
+    const std::vector<double>& pf_same() const { return pf_same_; }           // [2B]
+    const std::vector<double>& pf_circ() const { return pf_circ_; }           // [2B]
+    const std::vector<double>& cbk() const { return cbk_; }                   // [2BK]
+    const std::vector<double>& cbk_sqrt() const { return cbk_sqrt_; }         // [BK]
+    const hierarchy_graph& graph() const { return graph_; }
+
+    void set_hamiltonian(const double* h);
+    // top ADO to load at the start (instance::set_hierarchy_top, instance.cpp:483-495)
+    const std::vector<double>& top() const { return top_; }                   // [N*N*2]
+    void set_hierarchy_top(const complex_t* top);
+    void set_hierarchy_top();  // element [0][0] = 1 (linear absorption)
+
+    // descriptor of the kernel inputs for heom_create (points into this instance)
+    heom_desc desc() const;
+
+private:
+    void precompute(const config& cfg);
+
+    sites_to_states_mode mode_;
+    const hierarchy_graph& graph_;
+    int_t states_, matsubaras_, baths_, baths_per_state_, width_, depth_, matrices_, first_non_plus_;
+    std::vector<double> hamiltonian_, pf_same_sum_, pf_same_, pf_circ_, cbk_, cbk_sqrt_, top_;
+    std::vector<std::int32_t> coupling_;
+};
+
+}  // namespace heom

@@ -1,0 +1,240 @@
+// solver.hpp -- population_dynamics_solver over the HIP library, and the observers.
+//
+// Mirrors the reference's solver interface for this path
+// (dm-heom/include/heom/population_dynamics_solver.hpp:31-312 and
+// dm-heom/include/heom/solver_base.hpp:36-385): step_forward(n), hierarchy_top,
+// hierarchy, hierarchy_norm, hierarchy_mmult_left, update_hierarchy_mask,
+// update_hamiltonian, reset.  Instead of an OpenCL ODE_T / STEPPER_T pair, one
+// heom_handle (include/heom_hip.h) owns the device state; errors from the library are
+// rethrown as std::runtime_error, as the reference's ocl::error_handler throws.
+// The observers produce the reference's observation files
+// (matrix_{diagonal,complete}_observer.hpp, matrix_trace_observer.hpp,
+// hierarchy_norm_observer.hpp): one row per observation, a header naming the columns.
+// The text format of numbers (the un-vendored noma/num real_format) is not part of the
+// contract; it is "%.17g" here, tab separated.
+#pragma once
+
+#include <complex>
+#include <cstdio>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "heom_hip.h"
+#include "hierarchy.hpp"
+
+namespace heom {
+
+inline void hip_call(int rc)
+{
+    if (rc != HEOM_OK)
+        throw std::runtime_error(std::string("heom: ") + heom_last_error());
+}
+
+class population_dynamics_solver {
+public:
+    // heom_create from the instance, initial value = the instance's top ADO
+    population_dynamics_solver(const config& cfg, instance& inst)
+        : inst_(inst), step_size_(cfg.solver_step_size())
+    {
+        const heom_desc d = inst.desc();
+        hip_call(heom_create(&d, &h_));
+        reset();
+    }
+    ~population_dynamics_solver()
+    {
+        if (h_)
+            heom_destroy(h_);
+    }
+    population_dynamics_solver(const population_dynamics_solver&) = delete;
+    population_dynamics_solver& operator=(const population_dynamics_solver&) = delete;
+
+    // population_dynamics_solver::reset (population_dynamics_solver.hpp:161-175)
+    void reset()
+    {
+        hip_call(heom_clear_state(h_));
+        hip_call(heom_set_top(h_, inst_.top().data()));
+        step_count_ = 0;
+        time_ = 0.0;
+    }
+
+    // population_dynamics_solver::step_forward (population_dynamics_solver.hpp:233-265)
+    void step_forward(int_t n)
+    {
+        hip_call(heom_step_rk4(h_, step_size_, n));
+        step_count_ += n;
+        time_ += (double)n * step_size_;
+    }
+
+    // n steps, the top ADO recorded every observe_steps on the device; returns the
+    // rows (n / observe_steps + 1 matrices, the first = the current state)
+    std::vector<complex_t> propagate_observe(int_t n, int_t observe_steps)
+    {
+        const int_t N = inst_.states();
+        std::vector<complex_t> tops((std::size_t)((n / observe_steps + 1) * N * N));
+        hip_call(heom_propagate(h_, step_size_, n, observe_steps, reinterpret_cast<double*>(tops.data())));
+        step_count_ += n;
+        time_ += (double)n * step_size_;
+        return tops;
+    }
+
+    void hierarchy_top(complex_t* buffer) { hip_call(heom_get_top(h_, reinterpret_cast<double*>(buffer))); }
+    void hierarchy(complex_t* buffer) { hip_call(heom_get_state(h_, reinterpret_cast<double*>(buffer))); }
+    void hierarchy_norm(real_t* buffer, real_t threshold = -1.0)
+    {
+        hip_call(heom_hierarchy_norm(h_, threshold, buffer));
+    }
+    void update_hierarchy_mask(const std::uint8_t* mask) { hip_call(heom_set_mask(h_, mask)); }
+    void hierarchy_mmult_left(const complex_t* m)
+    {
+        hip_call(heom_mmult_left(h_, reinterpret_cast<const double*>(m)));
+    }
+    void update_hamiltonian(const real_t* h)
+    {
+        inst_.set_hamiltonian(h);
+        hip_call(heom_set_hamiltonian(h_, h));
+    }
+
+    real_t time() const { return time_; }
+    int_t step_count() const { return step_count_; }
+    instance& heom_instance() { return inst_; }
+    heom_handle* handle() { return h_; }
+    heom_timing_info timing()
+    {
+        heom_timing_info t{};
+        hip_call(heom_timing(h_, &t));
+        return t;
+    }
+
+private:
+    instance& inst_;
+    heom_handle* h_ = nullptr;
+    real_t step_size_;
+    int_t step_count_ = 0;
+    real_t time_ = 0.0;
+};
+
+// ---- observation output ----------------------------------------------------------
+
+inline std::string fmt_real(double v)
+{
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return buf;
+}
+
+constexpr char kDelimiter = '\t';
+
+// one observer: a header and a row writer over the top ADO (or the norms)
+class file_observer {
+public:
+    file_observer(const std::string& type, const std::string& filename, const instance& inst)
+        : type_(type), inst_(inst), out_(filename)
+    {
+        if (out_.fail())
+            throw std::runtime_error("Error: could not open output file: " + filename);
+        if (type != "matrix_diagonal" && type != "matrix_complete" && type != "matrix_trace_id" &&
+            type != "hierarchy_norm")
+            throw std::runtime_error("observation type '" + type + "' is not supported by this application");
+        write_header();
+    }
+
+    bool needs_norms() const { return type_ == "hierarchy_norm"; }
+
+    // write one row from the top ADO (and the norms for hierarchy_norm)
+    void observe(double time, const complex_t* top, const std::vector<double>* norms = nullptr)
+    {
+        const int_t N = inst_.states();
+        out_ << fmt_real(time);
+        if (type_ == "matrix_diagonal") {  // matrix_diagonal_observer.hpp:28-34
+            for (int_t i = 0; i < N; ++i)
+                out_ << kDelimiter << fmt_real(top[i * N + i].real());
+        } else if (type_ == "matrix_complete") {  // matrix_complete_observer.hpp
+            for (int_t e = 0; e < N * N; ++e)
+                out_ << kDelimiter << fmt_real(top[e].real()) << kDelimiter << fmt_real(top[e].imag());
+        } else if (type_ == "matrix_trace_id") {  // matrix_trace_observer.hpp (identity)
+            complex_t tr = 0.0;
+            for (int_t i = 0; i < N; ++i)
+                tr += top[i * N + i];
+            out_ << kDelimiter << fmt_real(tr.real()) << kDelimiter << fmt_real(tr.imag());
+        } else if (norms) {  // hierarchy_norm_observer.hpp
+            for (double v : *norms)
+                out_ << kDelimiter << fmt_real(v);
+        }
+        out_ << '\n';
+    }
+
+private:
+    void write_header()
+    {
+        const int_t N = inst_.states();
+        out_ << "time";
+        if (type_ == "matrix_diagonal") {
+            for (int_t i = 0; i < N; ++i)
+                out_ << kDelimiter << "site_" << i;
+        } else if (type_ == "matrix_complete") {
+            for (int_t i = 0; i < N; ++i)
+                for (int_t j = 0; j < N; ++j)
+                    out_ << kDelimiter << "elem_" << i << "_" << j << "_real" << kDelimiter << "elem_" << i << "_"
+                         << j << "_imag";
+        } else if (type_ == "matrix_trace_id") {
+            out_ << kDelimiter << "trace_real" << kDelimiter << "trace_imag";
+        } else {
+            const auto& g = inst_.graph();
+            for (int_t u = 0; u < g.nodes(); ++u) {
+                out_ << kDelimiter << "{";
+                for (int_t m = 0; m < g.width(); ++m)
+                    out_ << (m ? ", " : "") << g.tuples()[(std::size_t)(u * g.width() + m)];
+                out_ << "}";
+            }
+        }
+        out_ << '\n';
+    }
+
+    std::string type_;
+    const instance& inst_;
+    std::ofstream out_;
+};
+
+// D- (scaled by the tensor prefactor) and D+ = (D-)^T of the linear-absorption pseudo
+// pathway +{(-, L), (+, L)} with a ground state (dm-heom/include/heom/dipole_matrix.hpp:
+// 30-74, dm-heom/include/heom/dipole_pseudo_pathway.hpp:20-36,
+// dm-heom/src/heom/dipole_matrix.cpp:182-208): row 0 of D- holds the site dipoles
+// mu_i = direction_i[component] * strength_i.
+inline void dipole_matrices(const config& cfg, std::size_t tensor_index, std::vector<complex_t>& d_minus,
+                            std::vector<complex_t>& d_plus)
+{
+    const int_t sites = cfg.system_sites(), n = sites + 1;
+    const auto& comp = cfg.dipole_tensor_components().at(tensor_index);
+    d_minus.assign((std::size_t)(n * n), complex_t(0.0, 0.0));
+    d_plus.assign((std::size_t)(n * n), complex_t(0.0, 0.0));
+    for (int_t i = 0; i < sites; ++i) {
+        const double m0 = cfg.dipole_directions()[(std::size_t)i][(std::size_t)comp[0]] *
+                          cfg.dipole_strengths()[(std::size_t)i];
+        const double m1 = cfg.dipole_directions()[(std::size_t)i][(std::size_t)comp[1]] *
+                          cfg.dipole_strengths()[(std::size_t)i];
+        d_minus[(std::size_t)(i + 1)] = m0;           // row 0, column i + 1
+        d_plus[(std::size_t)((i + 1) * n)] = m1;      // transposed: row i + 1, column 0
+    }
+    const complex_t pre = cfg.dipole_tensor_prefactors().at(tensor_index);
+    for (auto& v : d_minus)
+        v *= pre;
+}
+
+// Tr(D top) as matrix_trace_observer::compute_trace (matrix_trace_observer.hpp:139-158)
+inline complex_t trace_product(const std::vector<complex_t>& d, const complex_t* top, int_t n)
+{
+    complex_t tr(0.0, 0.0);
+    for (int_t i = 0; i < n; ++i) {
+        complex_t v(0.0, 0.0);
+        for (int_t k = 0; k < n; ++k)
+            v += d[(std::size_t)(i * n + k)] * top[k * n + i];
+        tr += v;
+    }
+    return tr;
+}
+
+}  // namespace heom

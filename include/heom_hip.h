@@ -106,6 +106,9 @@ int heom_get_mask(heom_handle* handle, uint8_t* mask /* [M] */);
 int heom_set_state(heom_handle* handle, const double* state /* [M*N*N*2] */);
 /* solver_base::update_hierarchy_top (solver_base.hpp:283-289): overwrite ADO 0 only */
 int heom_set_top(heom_handle* handle, const double* top /* [N*N*2] */);
+/* instance::null_hierarchy (dm-heom/src/heom/instance.cpp:448-453) on the device:
+ * every ADO zero (with heom_set_top: a fresh initial value without a host-side state) */
+int heom_clear_state(heom_handle* handle);
 
 /* ---- outputs ------------------------------------------------------------------- */
 
