@@ -33,6 +33,9 @@ def parse_args():
     p.add_argument("--config", default="excitonic8_d8")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel of "
+                        "known size, so that profiler byte counters can be calibrated in the same run)")
     return p.parse_args()
 
 
@@ -99,6 +102,13 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev = h.timing().rk4_seconds - t_ev0  # HIP events on the library's stream
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # elementwise kernel: reads and writes copy_mib MiB
+        torch.cuda.synchronize()
+        del src, dst
     top = h.get_top()
     trace_err = abs(complex(top.trace()) - 1.0)
 
