@@ -12,9 +12,10 @@
 //     that stays exactly zero; edges that do not exist, or whose target is masked,
 //     point at slot M ("zero slot"), so gathers need no per-lane branches.
 //   * Neighbour tables are mode-major, one int2 per (mode, ADO):
-//     nb[m * Mp + q] = { plus index << 6 | n_m, minus index }, m = 0..W, so a term
-//     costs one 8-byte load; the extra mode W holds the zero slot everywhere and serves
-//     the padding entries of the per-element term lists.  Packing limits M to 2^26 - 1
+//     nb[m * Mp + q] = { plus index << 6 | n_m, minus index }, m = 0..W+1, so a term
+//     costs one 8-byte load; the extra rows W and W+1 hold the zero slot everywhere:
+//     row W serves the padding entries of the per-element term lists, and row W+1 lets
+//     the pipelined kernel stage rows in pairs.  Packing limits M to 2^26 - 1
 //     ADOs and n_m to 62.
 //
 // RK4 schedule (classical RK4, 12 state passes per step instead of 16): with
@@ -50,19 +51,25 @@ struct Term {
     double2 minus_c;
     double plus_c;
     int32_t mode;
-    int32_t pad;
+    int32_t info;  // bits 0-7: non-padding terms of this element (same in all its slots);
+                   // bit 8: plus_c != 0; bit 9: minus_c != 0
 };
+constexpr int32_t kTermHasPlus = 1 << 8;
+constexpr int32_t kTermHasMinus = 1 << 9;
 
 struct StageArgs {
     const double2* __restrict__ yin;   // stage input Y_s (neighbours are read from here)
     const double2* __restrict__ y0;    // y_n
     double2* __restrict__ pbuf;        // P = Y2 + 2 Y3 - y
     double2* __restrict__ yout;        // next stage input, y_{n+1}, or f (stage 0)
-    const int2* __restrict__ nb;       // [W+1][Mp] packed neighbour table
+    const int2* __restrict__ nb;       // [W+2][Mp] packed neighbour table
     const double2* __restrict__ pfss;  // [Mp] pf_same_sum
     int64_t mp;
-    int32_t ntiles;   // workgroups = Mp / 64
-    double c1;        // stage coefficient (h/2, h/2, h, h/6)
+    int32_t ntiles;       // Mp / 64
+    int32_t zero_slot;    // index of the zero slot (M)
+    int32_t table_units;  // (W + 2) / 2: pairs of table rows 0..W (pipelined kernel)
+    int32_t xcd_chunk;    // tiles per XCD chunk (chunk_tile)
+    double c1;            // stage coefficient (h/2, h/2, h, h/6)
 };
 
 // Element constants, read with wave-uniform indices; passed as separate restrict
@@ -84,13 +91,26 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c)  // c +
     return c;
 }
 
-// workgroup index -> tile index so that each XCD works on one contiguous range of
-// tiles (neighbour reads then hit that XCD's L2)
-__device__ __forceinline__ int xcd_tile(int b, int nt)
+// Tile order across XCDs.  Workgroups are dispatched round-robin over the XCDs
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  The tiles are cut into
+// chunks of `chunk` consecutive tiles; chunk k goes to XCD k % 8, so every XCD streams
+// one chunk at a time and all XCDs advance through the hierarchy together.  Small
+// chunks keep the XCDs' working sets close together (neighbour reads of far modes then
+// find data another XCD just streamed in the shared last-level cache); a chunk of
+// ntiles / 8 gives each XCD one contiguous range (neighbour reads of near modes stay in
+// its own L2).
+//   j-th tile of XCD x:
+__device__ __forceinline__ int chunk_tile(int x, int j, int chunk)
 {
-    const int per = nt / kNumXcd, rem = nt % kNumXcd;
-    const int x = b % kNumXcd, k = b / kNumXcd;
-    return x * per + min(x, rem) + k;
+    return (kNumXcd * (j / chunk) + x) * chunk + j % chunk;
+}
+
+// workgroup index -> tile index for one workgroup per tile: chunked over the largest
+// prefix made of whole rounds of kNumXcd chunks, identity on the rest (a bijection)
+__device__ __forceinline__ int xcd_tile(int b, int nt, int chunk)
+{
+    const int full = nt / (kNumXcd * chunk) * (kNumXcd * chunk);
+    return b < full ? chunk_tile(b % kNumXcd, b / kNumXcd, chunk) : b;
 }
 
 // value of `tab` held by lane `idx` (0..63) of this wavefront; all lanes must be active
@@ -102,27 +122,54 @@ __device__ __forceinline__ double lane_value(double tab, int idx)
     return __hiloint2double(hi, lo);
 }
 
-// RK4 stage epilogue for one element (see the schedule in the file header)
+// RK4 stage combination for one element (see the schedule in the file header):
+// s = stage input, f = RHS at s, ep = y_n (stages 2, 3) or P (stage 4); r0 goes to the
+// stage output, r1 (stage 2 only) to P.
+template <int STAGE>
+__device__ __forceinline__ void stage_values(double c1, double2 s, double2 f, double2 ep, double2& r0, double2& r1)
+{
+    if (STAGE == 0) {
+        r0 = f;
+    } else if (STAGE == 1) {
+        r0 = make_double2(fma(c1, f.x, s.x), fma(c1, f.y, s.y));
+    } else if (STAGE == 2) {
+        const double2 y3 = make_double2(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
+        r0 = y3;
+        r1 = make_double2(fma(2.0, y3.x, s.x) - ep.x, fma(2.0, y3.y, s.y) - ep.y);
+    } else if (STAGE == 3) {
+        r0 = make_double2(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
+    } else {
+        constexpr double third = 1.0 / 3.0;
+        r0 = make_double2(fma(ep.x + s.x, third, c1 * f.x), fma(ep.y + s.y, third, c1 * f.y));
+    }
+}
+
+// the epilogue operand of a stage: y_n for stages 2 and 3, P for stage 4
+template <int STAGE>
+__device__ __forceinline__ double2 stage_operand(const StageArgs& a, int64_t idx)
+{
+    if (STAGE == 2 || STAGE == 3)
+        return a.y0[idx];
+    if (STAGE == 4)
+        return a.pbuf[idx];
+    return make_double2(0.0, 0.0);
+}
+
+template <int STAGE>
+__device__ __forceinline__ void stage_store(const StageArgs& a, int64_t idx, double2 r0, double2 r1)
+{
+    a.yout[idx] = r0;
+    if (STAGE == 2)
+        a.pbuf[idx] = r1;
+}
+
+// RK4 stage epilogue for one element
 template <int STAGE>
 __device__ __forceinline__ void stage_epilogue(const StageArgs& a, int64_t idx, double2 s, double2 f)
 {
-    if (STAGE == 0) {
-        a.yout[idx] = f;
-    } else if (STAGE == 1) {
-        a.yout[idx] = make_double2(fma(a.c1, f.x, s.x), fma(a.c1, f.y, s.y));
-    } else if (STAGE == 2) {
-        const double2 y = a.y0[idx];
-        const double2 y3 = make_double2(fma(a.c1, f.x, y.x), fma(a.c1, f.y, y.y));
-        a.yout[idx] = y3;
-        a.pbuf[idx] = make_double2(fma(2.0, y3.x, s.x) - y.x, fma(2.0, y3.y, s.y) - y.y);
-    } else if (STAGE == 3) {
-        const double2 y = a.y0[idx];
-        a.yout[idx] = make_double2(fma(a.c1, f.x, y.x), fma(a.c1, f.y, y.y));
-    } else {
-        const double2 p = a.pbuf[idx];
-        constexpr double third = 1.0 / 3.0;
-        a.yout[idx] = make_double2(fma(p.x + s.x, third, a.c1 * f.x), fma(p.y + s.y, third, a.c1 * f.y));
-    }
+    double2 r0, r1 = make_double2(0.0, 0.0);
+    stage_values<STAGE>(a.c1, s, f, stage_operand<STAGE>(a, idx), r0, r1);
+    stage_store<STAGE>(a, idx, r0, r1);
 }
 
 // ---- launcher (stage_kernel.hip) -------------------------------------------------------
@@ -133,5 +180,20 @@ bool stage_supported(int N, int nterms);
 int stage_term_slots(int nterms);
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a, const ElementArgs& el,
                   hipStream_t s);
+
+// ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
+
+// modes of the neighbour table one LDS buffer of the pipelined kernel holds: two state
+// tiles (N*N x 64 complex128) and two table tiles (capacity x 64 int2) in 160 KiB,
+// with 4 KiB to spare, an even number, at most 64
+__host__ __device__ constexpr int pipe_table_capacity(int N)
+{
+    return (((163840 - 2048 * N * N) / 1024 - 4) < 64 ? ((163840 - 2048 * N * N) / 1024 - 4) : 64) & ~1;
+}
+// true if the pipelined kernel handles N states with W bath modes
+bool pipe_supported(int N, int W);
+// grid: `workgroups` persistent workgroups, a multiple of kNumXcd
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
+                            const ElementArgs& el, int workgroups, hipStream_t s);
 
 }  // namespace heom_hip

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -195,6 +196,9 @@ struct heom_handle {
     int32_t zero_slot = 0;
     bool real_h = false;
     int term_slots = 2;  // neighbour terms per element the stage kernel runs with
+    bool pipelined = false;  // stage_pipelined.hip (persistent) instead of stage_kernel.hip
+    int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
+    int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
     int64_t q_top = 0;   // internal index of uid 0
 
     std::vector<int32_t> coupling;  // [N * BM]
@@ -260,9 +264,15 @@ void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, 
     a.pfss = h->pfss.p;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
+    a.zero_slot = h->zero_slot;
+    a.table_units = (h->W + 2) / 2;
+    a.xcd_chunk = h->xcd_chunk;
     a.c1 = c1;
     ElementArgs el{h->hs.p, h->gam.p, h->terms.p};
-    heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+    if (h->pipelined)
+        heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
+    else
+        heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
     HIPC(hipGetLastError());
     h->timing.rhs_launches += 1;
 }
@@ -280,7 +290,7 @@ void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 
 void refold_tables(heom_handle* h)
 {
-    const int64_t count = (int64_t)(h->W + 1) * h->Mp;
+    const int64_t count = (int64_t)(h->W + 2) * h->Mp;
     const int threads = 256;
     const unsigned blocks = (unsigned)((count + threads - 1) / threads);
     hipLaunchKernelGGL(fold_mask, dim3(blocks), dim3(threads), 0, h->stream, h->nb_raw.p, h->nb.p, h->mask.p,
@@ -383,7 +393,10 @@ void build_coefficients(heom_handle* h)
             t.mode = W;
             t.plus_c = 0.0;
             t.minus_c = make_double2(0.0, 0.0);
-            flat[(size_t)e * h->term_slots + k] = k < (int)per[e].size() ? per[e][k] : t;
+            Term& dst = flat[(size_t)e * h->term_slots + k];
+            dst = k < (int)per[e].size() ? per[e][k] : t;
+            dst.info = (int32_t)per[e].size() | (dst.plus_c != 0.0 ? kTermHasPlus : 0) |
+                       (dst.minus_c.x != 0.0 || dst.minus_c.y != 0.0 ? kTermHasMinus : 0);
         }
     upload(h->terms, flat, h->stream);
     upload(h->gam, gam, h->stream);
@@ -437,6 +450,34 @@ void need(heom_handle* h)
 {
     if (!h)
         raise(HEOM_E_INVALID, "handle is NULL");
+}
+
+// The persistent pipelined stage kernel runs one workgroup per CU; it needs N >= 6 and
+// the neighbour-table slice of a tile in LDS.  HEOM_STAGE_KERNEL=tile in the
+// environment selects the one-workgroup-per-tile kernel instead.  The tile order
+// (chunk_tile in heom_device.hpp) defaults to one contiguous range per XCD for the
+// tile kernel and to XCD chunks of one tile per workgroup for the pipelined kernel;
+// HEOM_XCD_CHUNK=<tiles> overrides it.
+void choose_stage_kernel(heom_handle* h)
+{
+    const char* env = std::getenv("HEOM_STAGE_KERNEL");
+    const bool want_tile = env && std::string(env) == "tile";
+    h->pipelined = !want_tile && pipe_supported(h->N, h->W);
+    if (h->pipelined) {
+        int cus = 0;
+        HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+        const int wgs = std::max(1, std::min(cus, h->ntiles));
+        h->pipe_wgs = (wgs + kNumXcd - 1) / kNumXcd * kNumXcd;
+        h->xcd_chunk = h->pipe_wgs / kNumXcd;
+    } else {
+        h->xcd_chunk = std::max(1, h->ntiles / kNumXcd);
+    }
+    if (const char* c = std::getenv("HEOM_XCD_CHUNK")) {
+        const long v = std::strtol(c, nullptr, 10);
+        if (v < 1 || v > h->ntiles)
+            raise(HEOM_E_INVALID, "HEOM_XCD_CHUNK must be in [1, number of 64-ADO tiles]");
+        h->xcd_chunk = (int)v;
+    }
 }
 
 unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
@@ -504,7 +545,9 @@ int heom_create(const heom_desc* d, heom_handle** out)
         h->q_top = u2q[0];
 
         const int32_t zero = h->zero_slot;
-        std::vector<int2> nb((W + 1) * Mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
+        // rows 0..W-1 are the modes; rows W and W+1 hold the zero slot everywhere (row W
+        // serves padding terms, and the pipelined kernel loads rows in pairs)
+        std::vector<int2> nb((W + 2) * Mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
         std::vector<double2> pfss(Mp, make_double2(0.0, 0.0));
         for (int64_t q = 0; q < M; ++q) {
             const int64_t u = order[q];
@@ -535,6 +578,7 @@ int heom_create(const heom_desc* d, heom_handle** out)
         }
         build_coefficients(h);
         set_hamiltonian(h, d->hamiltonian);
+        choose_stage_kernel(h);
         HIPC(hipStreamSynchronize(h->stream));
     });
     if (rc != HEOM_OK) {
@@ -809,6 +853,16 @@ int heom_rk4_step_bytes(heom_handle* h, int64_t* bytes)
         if (!bytes)
             raise(HEOM_E_INVALID, "bytes is NULL");
         *bytes = (int64_t)16 * h->M * h->nel * (int64_t)sizeof(double2);
+    });
+}
+
+int heom_stage_kernel(heom_handle* h, int* kind)
+{
+    return guard([&] {
+        need(h);
+        if (!kind)
+            raise(HEOM_E_INVALID, "kind is NULL");
+        *kind = h->pipelined ? HEOM_STAGE_PIPELINED : HEOM_STAGE_TILE;
     });
 }
 

@@ -1,0 +1,293 @@
+// stage_pipelined.hip -- persistent, software-pipelined form of the stage kernel.
+//
+// Same RHS and RK4 stage arithmetic as stage_kernel.hip (dm-heom/cl/heom/heom_ode.cl
+// :24-451 with rk_weighted_add.cl:19-106 fused in), organised so that the streaming
+// loads of one ADO tile overlap the neighbour gathers and arithmetic of the previous
+// one:
+//   * one workgroup per CU (N * ceil(N/4) wavefronts) walks a run of 64-ADO tiles: the
+//     XCD's workgroups take turns through its sequence of tiles (chunk_tile), so they
+//     move through the hierarchy together;
+//   * two LDS buffers (A, B), each holding a tile's N*N element planes and its slice of
+//     the mode-major neighbour table; while the tile in one buffer is computed, the
+//     next tile streams into the other with direct-to-LDS loads (no registers);
+//   * the neighbour indices come from the LDS table slice (modes 0..W, W being the
+//     all-zero-slot row that padding terms name);
+//   * a tile's results stay in registers across the barrier and are stored at the start
+//     of the next tile, so no wave waits for its own stores to complete.
+// The two buffers are separate LDS objects used with compile-time selection (the loop
+// is unrolled by two); that lets the compiler see that loads into one buffer do not
+// alias reads of the other, so it does not wait for the prefetch before computing.
+#include "heom_device.hpp"
+
+namespace heom_hip {
+
+namespace {
+
+#define HEOM_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
+
+template <int N>
+struct Shape {
+    static constexpr int CW = kStageColumns;
+    static constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
+    static constexpr int NWAVE = N * NJW;
+    static constexpr int TC = pipe_table_capacity(N);
+};
+
+// issue the loads of tile `tile` into one LDS buffer: this wave's own elements and its
+// share of the table units (two modes x 64 ADOs per 16-byte-per-lane load)
+template <int N>
+__device__ __forceinline__ void prefetch_tile(const StageArgs& a, int tile, int w, int i, int jb, int lane,
+                                              double2* sig, int2* tab)
+{
+    using S = Shape<N>;
+    const int64_t mp = a.mp;
+    const uint32_t q = (uint32_t)tile * kTile + lane;
+#pragma unroll
+    for (int c = 0; c < S::CW; ++c) {
+        const int j = jb + c;
+        if (j < N) {
+            const int e = i * N + j;
+            __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)e * mp + q), HEOM_LDS_PTR(sig + e * kTile),
+                                             16, 0, 0);
+        }
+    }
+    const int half = lane >> 5, pair = (lane & 31) * 2;
+    for (int u = w; u < a.table_units; u += S::NWAVE) {
+        const int m = 2 * u + half;
+        __builtin_amdgcn_global_load_lds((const void*)(a.nb + (int64_t)m * mp + (int64_t)tile * kTile + pair),
+                                         HEOM_LDS_PTR(tab + 2 * u * kTile), 16, 0, 0);
+    }
+}
+
+// RHS of one tile from LDS (own elements, table) plus neighbour gathers; the stage
+// results of the wave's elements land in r0 (and r1 for stage 2)
+template <int N, int NT, int STAGE, bool REALH>
+__device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* __restrict__ hs,
+                                             const double2* __restrict__ gam, const Term* __restrict__ terms,
+                                             int tile, int i, int jb, int lane, double sq_lane, const double2* sig,
+                                             const int2* tab, double2 (&r0)[kStageColumns],
+                                             double2 (&r1)[kStageColumns])
+{
+    constexpr int CW = kStageColumns;
+    const int64_t mp = a.mp;
+    const uint32_t q = (uint32_t)tile * kTile + lane;
+    // The element constants are the same for every tile.  An offset the compiler cannot
+    // see through keeps their (scalar, cached) loads inside the tile loop instead of
+    // hoisting all of them into registers for the whole kernel.
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    hs += z;
+    gam += z;
+    terms += z;
+
+    const double2 S = a.pfss[q];
+
+    // (sigma H)_ij of the wave's columns, row i of sigma read once from the tile
+    double2 sh[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+        sh[c] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double2 sik = sig[(i * N + k) * kTile + lane];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            if (jb + c < N) {
+                const double2 hkj = hs[k * N + jb + c];
+                if (REALH) {
+                    sh[c].x = fma(sik.x, hkj.x, sh[c].x);
+                    sh[c].y = fma(sik.y, hkj.x, sh[c].y);
+                } else {
+                    sh[c] = cfma(sik, hkj, sh[c]);
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const int j = jb + c;
+        if (j >= N)
+            break;
+        const int e = i * N + j;
+        const int64_t idx = (int64_t)e * mp + q;
+        const double2* plane = a.yin + (int64_t)e * mp;
+        const Term* te = terms + e * NT;
+
+        // neighbour indices from the LDS table (padding terms name mode W, whose row is
+        // the zero slot), then the gathers; missing and masked neighbours read the zero
+        // slot, so the code is branch-free
+        int2 nbv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k)
+            nbv[k] = tab[te[k].mode * kTile + lane];
+        double2 xp[NT], xm[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            xp[k] = plane[(uint32_t)nbv[k].x >> 6];
+            xm[k] = plane[(uint32_t)nbv[k].y];
+        }
+        const double2 ep = stage_operand<STAGE>(a, idx);
+
+        // commutator c = (sigma H)_ij - sum_k H_ik sigma_kj
+        double2 cm = sh[c];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double2 skj = sig[(k * N + j) * kTile + lane];
+            const double2 hik = hs[i * N + k];
+            if (REALH) {
+                cm.x = fma(-hik.x, skj.x, cm.x);
+                cm.y = fma(-hik.x, skj.y, cm.y);
+            } else {
+                cm = cfma(make_double2(-hik.x, -hik.y), skj, cm);
+            }
+        }
+        const double2 s = sig[e * kTile + lane];
+        const double2 g = gam[e];
+        double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), s, make_double2(-cm.y, cm.x));
+
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int v = nbv[k].x & 63;
+            const double cp = te[k].plus_c * lane_value(sq_lane, v + 1);
+            const double sm = lane_value(sq_lane, v);
+            f.x = fma(-cp, xp[k].y, f.x);
+            f.y = fma(cp, xp[k].x, f.y);
+            f = cfma(te[k].minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
+        }
+        stage_values<STAGE>(a.c1, s, f, ep, r0[c], r1[c]);
+#ifdef HEOM_PIPE_COLUMN_FENCE
+        __builtin_amdgcn_sched_barrier(0);  // columns do not interleave: bounded registers
+#endif
+    }
+}
+
+template <int N, int STAGE>
+__device__ __forceinline__ void store_tile(const StageArgs& a, int tile, int i, int jb, int lane,
+                                           const double2 (&r0)[kStageColumns], const double2 (&r1)[kStageColumns])
+{
+    const uint32_t q = (uint32_t)tile * kTile + lane;
+#pragma unroll
+    for (int c = 0; c < kStageColumns; ++c)
+        if (jb + c < N)
+            stage_store<STAGE>(a, (int64_t)(i * N + jb + c) * a.mp + q, r0[c], r1[c]);
+}
+
+template <int N, int NT, int STAGE, bool REALH>
+__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs a, const double2* __restrict__ hs,
+                                                                       const double2* __restrict__ gam,
+                                                                       const Term* __restrict__ terms)
+{
+    using S = Shape<N>;
+    __shared__ __align__(16) double2 sigA[N * N * kTile];
+    __shared__ __align__(16) double2 sigB[N * N * kTile];
+    __shared__ __align__(16) int2 tabA[S::TC * kTile];
+    __shared__ __align__(16) int2 tabB[S::TC * kTile];
+
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = w / S::NJW;
+    const int jb = (w % S::NJW) * S::CW;
+
+    // tiles of this workgroup: the j-th tile of XCD x is chunk_tile(x, j, chunk); the gx
+    // workgroups of the XCD take j = g, g + gx, ...  The sequence increases with j, so
+    // the first tile past the end ends the walk.
+    const int x = blockIdx.x % kNumXcd, g = blockIdx.x / kNumXcd, gx = gridDim.x / kNumXcd;
+    const int chunk = a.xcd_chunk, nt = a.ntiles;
+    int j = g;
+    int t = chunk_tile(x, j, chunk);
+    if (t >= nt)
+        return;  // the whole workgroup leaves together
+    const double sq_lane = sqrt((double)lane);
+
+    double2 r0[kStageColumns], r1[kStageColumns];
+#pragma unroll
+    for (int c = 0; c < kStageColumns; ++c)
+        r0[c] = r1[c] = make_double2(0.0, 0.0);
+
+    prefetch_tile<N>(a, t, w, i, jb, lane, sigA, tabA);
+    __syncthreads();  // waits for the loads of this wave, then for every wave
+
+    int prev = -1;
+    for (;;) {
+        // tile t is in A; the next one goes to B
+        j += gx;
+        int tn = chunk_tile(x, j, chunk);
+        if (prev >= 0)
+            store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
+        if (tn < nt)
+            prefetch_tile<N>(a, tn, w, i, jb, lane, sigB, tabB);
+        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, t, i, jb, lane, sq_lane, sigA, tabA, r0, r1);
+        prev = t;
+        t = tn;
+        __syncthreads();  // B is complete, and nobody reads A any more
+        if (t >= nt)
+            break;
+
+        // tile t is in B; the next one goes to A
+        j += gx;
+        tn = chunk_tile(x, j, chunk);
+        store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
+        if (tn < nt)
+            prefetch_tile<N>(a, tn, w, i, jb, lane, sigA, tabA);
+        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, t, i, jb, lane, sq_lane, sigB, tabB, r0, r1);
+        prev = t;
+        t = tn;
+        __syncthreads();
+        if (t >= nt)
+            break;
+    }
+    store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
+}
+
+template <int N, int NT, bool REALH>
+void launch_t(int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+{
+    const dim3 grid(wgs), block(64 * stage_waves(N));
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    }
+}
+
+template <int N, bool REALH>
+void launch_n(int nt, int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+{
+    switch (nt) {
+    case 2: launch_t<N, 2, REALH>(stage, a, el, wgs, st); break;
+    case 4: launch_t<N, 4, REALH>(stage, a, el, wgs, st); break;
+    default: launch_t<N, 8, REALH>(stage, a, el, wgs, st); break;
+    }
+}
+
+template <bool REALH>
+void launch_r(int N, int nt, int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+{
+    switch (N) {
+    case 6: launch_n<6, REALH>(nt, stage, a, el, wgs, st); break;
+    case 7: launch_n<7, REALH>(nt, stage, a, el, wgs, st); break;
+    default: launch_n<8, REALH>(nt, stage, a, el, wgs, st); break;
+    }
+}
+
+}  // namespace
+
+bool pipe_supported(int N, int W)
+{
+    // wide enough workgroups to carry a CU on their own, and the table fits the LDS buffer
+    return N >= 6 && N <= kMaxStates && ((W + 2) & ~1) <= pipe_table_capacity(N);
+}
+
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
+                            const ElementArgs& el, int workgroups, hipStream_t s)
+{
+    if (real_h)
+        launch_r<true>(N, nterm_slots, stage, a, el, workgroups, s);
+    else
+        launch_r<false>(N, nterm_slots, stage, a, el, workgroups, s);
+}
+
+}  // namespace heom_hip

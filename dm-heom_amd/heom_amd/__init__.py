@@ -36,8 +36,10 @@ EXPORTED = [
     "heom_clear_state",
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
-    "heom_rk4_step_bytes",
+    "heom_rk4_step_bytes", "heom_stage_kernel",
 ]
+HEOM_STAGE_TILE = 0
+HEOM_STAGE_PIPELINED = 1
 
 
 class HeomError(RuntimeError):
@@ -102,6 +104,7 @@ def load():
             "heom_timing": [vp, ctypes.POINTER(TimingInfo)],
             "heom_device_bytes": [vp, ctypes.POINTER(i64)],
             "heom_rk4_step_bytes": [vp, ctypes.POINTER(i64)],
+            "heom_stage_kernel": [vp, ctypes.POINTER(ctypes.c_int)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -293,6 +296,12 @@ class Handle:
         b = ctypes.c_int64()
         _check(load().heom_rk4_step_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+    def stage_kernel(self) -> str:
+        """"pipelined" or "tile": the stage kernel this handle runs."""
+        k = ctypes.c_int()
+        _check(load().heom_stage_kernel(self._h, ctypes.byref(k)))
+        return "pipelined" if k.value == HEOM_STAGE_PIPELINED else "tile"
 
 
 # --------------------------------------------------------------------------------------

@@ -153,6 +153,12 @@ int heom_device_bytes(heom_handle* handle, int64_t* bytes);
 /* algorithmic HBM bytes of one RK4 step of the whole hierarchy
  * (16 state passes x M x N^2 x sizeof(complex)) */
 int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
+/* which stage kernel the handle runs (no counterpart in the reference; for tests and
+ * benchmarks): HEOM_STAGE_TILE = one workgroup per 64-ADO tile, HEOM_STAGE_PIPELINED =
+ * persistent workgroups streaming tiles through LDS (N >= 6 and few bath modes).  The
+ * environment variable HEOM_STAGE_KERNEL=tile at heom_create forces the tile kernel. */
+enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1 };
+int heom_stage_kernel(heom_handle* handle, int* kind);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,17 @@ pytestmark = pytest.mark.gpu
 heom_amd = pytest.importorskip("heom_amd")
 
 
+@pytest.fixture(params=["pipelined", "tile"], autouse=True)
+def stage_kernel(request, monkeypatch):
+    """Every test runs with both stage kernels: the persistent pipelined one (chosen by
+    default where it applies: N >= 6, few modes) and the one-workgroup-per-tile one."""
+    if request.param == "tile":
+        monkeypatch.setenv("HEOM_STAGE_KERNEL", "tile")
+    else:
+        monkeypatch.delenv("HEOM_STAGE_KERNEL", raising=False)
+    return request.param
+
+
 def make_instance(N, B, K, D, coupling, seed=0, complex_h=False, omega=None, temperature=300.0):
     rng = np.random.default_rng(seed)
     H = np.diag(rng.uniform(12000, 12600, N)).astype(np.complex128)
@@ -34,20 +45,31 @@ def make_instance(N, B, K, D, coupling, seed=0, complex_h=False, omega=None, tem
                       temperature=temperature)
 
 
-def gpu_rhs(inst, y, mask=None):
+def expected_kernel(inst, requested):
+    """The pipelined kernel holds two N*N x 64 state tiles and two (W+1)-row table tiles
+    (rounded to an even row count) in 160 KiB of LDS with 4 KiB spare."""
+    N, W = inst.states, inst.width
+    cap = min(64, (163840 - 2048 * N * N) // 1024 - 4) & ~1
+    ok = requested == "pipelined" and N >= 6 and ((W + 2) & ~1) <= cap
+    return "pipelined" if ok else "tile"
+
+
+def gpu_rhs(inst, y, mask=None, kernel=None):
     with heom_amd.Handle.from_instance(inst) as h:
+        if kernel is not None:
+            assert h.stage_kernel() == expected_kernel(inst, kernel)
         if mask is not None:
             h.set_mask(mask)
         h.set_state(y)
         return h.rhs()
 
 
-def check_rhs(inst, seed=1, mask=None):
+def check_rhs(inst, seed=1, mask=None, kernel=None):
     y = random_state(inst, seed)
     if mask is not None:
         inst.mask[:] = mask
     ref = inst.rhs(y)
-    got = gpu_rhs(inst, y, mask)
+    got = gpu_rhs(inst, y, mask, kernel)
     inst.mask[:] = O.MASK_TRUE
     err = max_abs_rel(got, ref)
     print(f"rhs N={inst.states} M={inst.matrices} W={inst.width}: max rel err {err:.3e}", flush=True)
@@ -60,21 +82,21 @@ def check_rhs(inst, seed=1, mask=None):
 
 
 @pytest.mark.parametrize("name,depth", [("excitonic7_d4", None), ("excitonic8_d8", 3), ("excitonic8_d6", None)])
-def test_rhs_workloads(oracle_mod, name, depth):
+def test_rhs_workloads(oracle_mod, stage_kernel, name, depth):
     _, inst = instance(name, depth=depth)
-    check_rhs(inst)
+    check_rhs(inst, kernel=stage_kernel)
 
 
-def test_rhs_fmo_example(oracle_mod):
+def test_rhs_fmo_example(oracle_mod, stage_kernel):
     cfg = O.parse_config(__import__("helpers").FMO_DIR + "/fmo_population_dynamics_77K.cfg")
     inst = O.instance_from_config(cfg)
-    check_rhs(inst, seed=3)
+    check_rhs(inst, seed=3, kernel=stage_kernel)
 
 
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8])
-def test_rhs_all_state_counts(oracle_mod, N):
+def test_rhs_all_state_counts(oracle_mod, stage_kernel, N):
     inst = make_instance(N, N, 2, 3, [[i] for i in range(N)], seed=N)
-    check_rhs(inst, seed=N)
+    check_rhs(inst, seed=N, kernel=stage_kernel)
 
 
 def test_rhs_complex_hamiltonian_and_omega(oracle_mod):
@@ -153,13 +175,14 @@ def test_mmult_left(oracle_mod):
     assert max_abs_rel(got, ref) < 1e-14
 
 
-def test_rk4_populations_excitonic7(oracle_mod):
+def test_rk4_populations_excitonic7(oracle_mod, stage_kernel):
     """The full 7-site depth-4 run: 1000 RK4 steps of 1 fs, observed every 10 steps."""
     cfg, inst = instance("excitonic7_d4")
     y0 = inst.state_with_top(cfg.rho_init)
     y = y0.copy()
     ref_tops = inst.propagate(y, cfg.step_size, cfg.steps, cfg.observe_steps)
     with heom_amd.Handle.from_instance(inst) as h:
+        assert h.stage_kernel() == expected_kernel(inst, stage_kernel)
         h.set_state(y0)
         tops = h.propagate(cfg.step_size, cfg.steps, cfg.observe_steps)
         final = h.get_state()
@@ -195,13 +218,14 @@ def test_step_rk4_matches_propagate(oracle_mod):
     assert t.rk4_steps >= 80 and t.rk4_seconds > 0
 
 
-def test_excitonic8_full_size_one_step(oracle_mod):
+def test_excitonic8_full_size_one_step(oracle_mod, stage_kernel):
     """735,471 ADOs: one RK4 step against the oracle, then invariants after 10 more."""
     cfg, inst = instance("excitonic8_d8")
     y = random_state(inst, 21)
     ref = y.copy()
     inst.rk4_step(ref, cfg.step_size)
     with heom_amd.Handle.from_instance(inst) as h:
+        assert h.stage_kernel() == expected_kernel(inst, stage_kernel)
         h.set_state(y)
         h.step_rk4(cfg.step_size, 1)
         got = h.get_state()
