@@ -69,6 +69,7 @@ struct StageArgs {
     int32_t zero_slot;    // index of the zero slot (M)
     int32_t table_units;  // (W + 2) / 2: pairs of table rows 0..W (pipelined kernel)
     int32_t xcd_chunk;    // tiles per XCD chunk (chunk_tile)
+    int32_t reverse;      // walk the tiles from the last one (tile t -> ntiles - 1 - t)
     double c1;            // stage coefficient (h/2, h/2, h, h/6)
 };
 
@@ -145,22 +146,39 @@ __device__ __forceinline__ void stage_values(double c1, double2 s, double2 f, do
 }
 
 // the epilogue operand of a stage: y_n for stages 2 and 3, P for stage 4
+// Streaming accesses with no reuse inside a stage (the stage operand y_n or P, and the
+// stage outputs) carry the non-temporal hint, so that they do not push out of the
+// caches the state lines that neighbour reads of other tiles will still ask for.
+typedef double heom_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 nt_load(const double2* p)
+{
+    const heom_d2v v = __builtin_nontemporal_load((const heom_d2v*)p);
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void nt_store(double2 v, double2* p)
+{
+    heom_d2v w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, (heom_d2v*)p);
+}
+
 template <int STAGE>
 __device__ __forceinline__ double2 stage_operand(const StageArgs& a, int64_t idx)
 {
     if (STAGE == 2 || STAGE == 3)
-        return a.y0[idx];
+        return nt_load(a.y0 + idx);
     if (STAGE == 4)
-        return a.pbuf[idx];
+        return nt_load(a.pbuf + idx);
     return make_double2(0.0, 0.0);
 }
 
 template <int STAGE>
 __device__ __forceinline__ void stage_store(const StageArgs& a, int64_t idx, double2 r0, double2 r1)
 {
-    a.yout[idx] = r0;
+    nt_store(r0, a.yout + idx);
     if (STAGE == 2)
-        a.pbuf[idx] = r1;
+        nt_store(r1, a.pbuf + idx);
 }
 
 // RK4 stage epilogue for one element
@@ -190,8 +208,8 @@ __host__ __device__ constexpr int pipe_table_capacity(int N)
 {
     return (((163840 - 2048 * N * N) / 1024 - 4) < 64 ? ((163840 - 2048 * N * N) / 1024 - 4) : 64) & ~1;
 }
-// true if the pipelined kernel handles N states with W bath modes
-bool pipe_supported(int N, int W);
+// true if the pipelined kernel handles N states with W bath modes and Mp ADO slots
+bool pipe_supported(int N, int W, int64_t mp);
 // grid: `workgroups` persistent workgroups, a multiple of kNumXcd
 void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
                             const ElementArgs& el, int workgroups, hipStream_t s);

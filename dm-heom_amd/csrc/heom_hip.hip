@@ -253,7 +253,7 @@ void resolve_timing(heom_handle* h)
     h->pending_steps = 0;
 }
 
-void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, double c1)
+void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, double c1, bool reverse = false)
 {
     StageArgs a{};
     a.yin = yin;
@@ -267,6 +267,7 @@ void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, 
     a.zero_slot = h->zero_slot;
     a.table_units = (h->W + 2) / 2;
     a.xcd_chunk = h->xcd_chunk;
+    a.reverse = reverse ? 1 : 0;
     a.c1 = c1;
     ElementArgs el{h->hs.p, h->gam.p, h->terms.p};
     if (h->pipelined)
@@ -281,10 +282,12 @@ void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, 
 void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 {
     for (int64_t s = 0; s < nsteps; ++s) {
-        launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt);
-        launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt);
-        launch_stage(h, 3, h->yb.p, h->ya.p, dt);
-        launch_stage(h, 4, h->ya.p, h->y.p, dt / 6.0);
+        // stages alternate their tile walk, so that each starts where the previous one
+        // ended, on the part of its input that is still in the last-level cache
+        launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt, false);
+        launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt, true);
+        launch_stage(h, 3, h->yb.p, h->ya.p, dt, false);
+        launch_stage(h, 4, h->ya.p, h->y.p, dt / 6.0, true);
     }
 }
 
@@ -462,7 +465,7 @@ void choose_stage_kernel(heom_handle* h)
 {
     const char* env = std::getenv("HEOM_STAGE_KERNEL");
     const bool want_tile = env && std::string(env) == "tile";
-    h->pipelined = !want_tile && pipe_supported(h->N, h->W);
+    h->pipelined = !want_tile && pipe_supported(h->N, h->W, h->Mp);
     if (h->pipelined) {
         int cus = 0;
         HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));

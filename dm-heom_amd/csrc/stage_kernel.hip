@@ -41,7 +41,8 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, co
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = w / NJW;
     const int jb = (w % NJW) * CW;
-    const int tile = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
+    const int t0 = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
+    const int tile = a.reverse ? a.ntiles - 1 - t0 : t0;
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
 

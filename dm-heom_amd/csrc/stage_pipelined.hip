@@ -25,6 +25,9 @@ namespace {
 
 #define HEOM_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
 
+// cache policy of the table loads: non-temporal (each table row is read once a stage)
+constexpr int kTableCachePolicy = 2;
+
 template <int N>
 struct Shape {
     static constexpr int CW = kStageColumns;
@@ -51,58 +54,87 @@ __device__ __forceinline__ void prefetch_tile(const StageArgs& a, int tile, int 
                                              16, 0, 0);
         }
     }
-    const int half = lane >> 5, pair = (lane & 31) * 2;
-    for (int u = w; u < a.table_units; u += S::NWAVE) {
-        const int m = 2 * u + half;
-        __builtin_amdgcn_global_load_lds((const void*)(a.nb + (int64_t)m * mp + (int64_t)tile * kTile + pair),
-                                         HEOM_LDS_PTR(tab + 2 * u * kTile), 16, 0, 0);
+    // table rows in pairs: lanes 0-31 load row 2u, lanes 32-63 row 2u+1, two ADOs per
+    // lane; byte offsets fit in 32 bits (pipe_supported)
+    const char* nb = (const char*)a.nb;
+    const uint32_t lane_off = (uint32_t)(((lane >> 5) * mp + (lane & 31) * 2) * sizeof(int2));
+#pragma unroll
+    for (int r = 0; r < (S::TC / 2 + S::NWAVE - 1) / S::NWAVE; ++r) {
+        const int u = w + r * S::NWAVE;
+        if (u < a.table_units) {
+            const uint32_t off = (uint32_t)((2 * u * mp + (int64_t)tile * kTile) * sizeof(int2)) + lane_off;
+            __builtin_amdgcn_global_load_lds((const void*)(nb + off), HEOM_LDS_PTR(tab + 2 * u * kTile), 16, 0,
+                                             kTableCachePolicy);
+        }
     }
 }
 
 // RHS of one tile from LDS (own elements, table) plus neighbour gathers; the stage
-// results of the wave's elements land in r0 (and r1 for stage 2)
-template <int N, int NT, int STAGE, bool REALH>
+// results of the wave's elements land in r0 (and r1 for stage 2).  `issue` runs right
+// after the first column's gathers are in flight: the previous tile's stores and the
+// next tile's prefetch go out there, behind those gathers, so that waiting for the
+// gathers does not mean waiting for the prefetch (loads complete in issue order).
+template <int N, int NT, int STAGE, bool REALH, typename Issue>
 __device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* __restrict__ hs,
                                              const double2* __restrict__ gam, const Term* __restrict__ terms,
                                              int tile, int i, int jb, int lane, double sq_lane, const double2* sig,
                                              const int2* tab, double2 (&r0)[kStageColumns],
-                                             double2 (&r1)[kStageColumns])
+                                             double2 (&r1)[kStageColumns], Issue&& issue)
 {
     constexpr int CW = kStageColumns;
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
-    // The element constants are the same for every tile.  An offset the compiler cannot
-    // see through keeps their (scalar, cached) loads inside the tile loop instead of
-    // hoisting all of them into registers for the whole kernel.
-    int z;
-    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-    hs += z;
-    gam += z;
-    terms += z;
-
+    // The element constants are the same for every tile and column block.  Offsets the
+    // compiler cannot see through keep their (scalar, cached) loads next to the column
+    // that uses them instead of hoisting all of them into scalar registers at once.
+    auto opaque = [](const auto* p) {
+        int z;
+        asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+        return p + z;
+    };
     const double2 S = a.pfss[q];
 
-    // (sigma H)_ij of the wave's columns, row i of sigma read once from the tile
-    double2 sh[CW];
+    // Column c's neighbour indices (from the LDS table; padding terms name mode W,
+    // whose row is the zero slot) and its gathers (missing and masked neighbours read
+    // the zero slot, so the code is branch-free).  Column 0's gathers go out before the
+    // previous tile's stores and the next tile's prefetch (`issue`); each later column
+    // issues its own at its start, and the commutator from LDS covers their latency.
+    struct Loads {
+        int2 nbv[NT];
+        double2 xp[NT], xm[NT];
+    } L[CW];
+    // a neighbour inside this tile is read from the LDS copy of the tile; the others
+    // from the element plane (in-tile lanes read the zero slot there, one cached line)
+    const uint32_t zero = (uint32_t)a.zero_slot;
+    auto neighbour = [&](const double2* plane, const double2* tile_plane, uint32_t p) {
+        const bool in = (p >> 6) == (uint32_t)tile;
+        const double2 g = plane[in ? zero : p];
+        const double2 l = tile_plane[p & 63];
+        return in ? l : g;
+    };
+    auto load = [&](int c) {
+        const int e = i * N + jb + c;
+        const double2* plane = a.yin + (int64_t)e * mp;
+        const Term* te = opaque(terms) + e * NT;
 #pragma unroll
-    for (int c = 0; c < CW; ++c)
-        sh[c] = make_double2(0.0, 0.0);
+        for (int k = 0; k < NT; ++k)
+            L[c].nbv[k] = tab[te[k].mode * kTile + lane];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const double2 sik = sig[(i * N + k) * kTile + lane];
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-            if (jb + c < N) {
-                const double2 hkj = hs[k * N + jb + c];
-                if (REALH) {
-                    sh[c].x = fma(sik.x, hkj.x, sh[c].x);
-                    sh[c].y = fma(sik.y, hkj.x, sh[c].y);
-                } else {
-                    sh[c] = cfma(sik, hkj, sh[c]);
-                }
-            }
+        for (int k = 0; k < NT; ++k) {
+#ifdef HEOM_LDS_NEIGHBOURS
+            L[c].xp[k] = neighbour(plane, sig + e * kTile, (uint32_t)L[c].nbv[k].x >> 6);
+            L[c].xm[k] = neighbour(plane, sig + e * kTile, (uint32_t)L[c].nbv[k].y);
+#else
+            L[c].xp[k] = plane[(uint32_t)L[c].nbv[k].x >> 6];
+            L[c].xm[k] = plane[(uint32_t)L[c].nbv[k].y];
+#endif
         }
-    }
+    };
+
+    load(0);
+    __builtin_amdgcn_sched_barrier(0);
+    issue();
+    __builtin_amdgcn_sched_barrier(0);
 
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
@@ -110,55 +142,42 @@ __device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* 
         if (j >= N)
             break;
         const int e = i * N + j;
-        const int64_t idx = (int64_t)e * mp + q;
-        const double2* plane = a.yin + (int64_t)e * mp;
-        const Term* te = terms + e * NT;
+        const double2 ep = stage_operand<STAGE>(a, (int64_t)e * mp + q);
+        if (c > 0)
+            load(c);
+        const Term* te = opaque(terms) + e * NT;
+        const double2* hc = opaque(hs);
 
-        // neighbour indices from the LDS table (padding terms name mode W, whose row is
-        // the zero slot), then the gathers; missing and masked neighbours read the zero
-        // slot, so the code is branch-free
-        int2 nbv[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k)
-            nbv[k] = tab[te[k].mode * kTile + lane];
-        double2 xp[NT], xm[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            xp[k] = plane[(uint32_t)nbv[k].x >> 6];
-            xm[k] = plane[(uint32_t)nbv[k].y];
-        }
-        const double2 ep = stage_operand<STAGE>(a, idx);
-
-        // commutator c = (sigma H)_ij - sum_k H_ik sigma_kj
-        double2 cm = sh[c];
+        // commutator c = sum_k sigma_ik H_kj - H_ik sigma_kj, row i and column j from the tile
+        double2 cm = make_double2(0.0, 0.0);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
+            const double2 sik = sig[(i * N + k) * kTile + lane];
             const double2 skj = sig[(k * N + j) * kTile + lane];
-            const double2 hik = hs[i * N + k];
+            const double2 hkj = hc[k * N + j];
+            const double2 hik = hc[i * N + k];
             if (REALH) {
-                cm.x = fma(-hik.x, skj.x, cm.x);
-                cm.y = fma(-hik.x, skj.y, cm.y);
+                cm.x = fma(sik.x, hkj.x, fma(-hik.x, skj.x, cm.x));
+                cm.y = fma(sik.y, hkj.x, fma(-hik.x, skj.y, cm.y));
             } else {
+                cm = cfma(sik, hkj, cm);
                 cm = cfma(make_double2(-hik.x, -hik.y), skj, cm);
             }
         }
         const double2 s = sig[e * kTile + lane];
-        const double2 g = gam[e];
+        const double2 g = opaque(gam)[e];
         double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), s, make_double2(-cm.y, cm.x));
 
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int v = nbv[k].x & 63;
+            const int v = L[c].nbv[k].x & 63;
             const double cp = te[k].plus_c * lane_value(sq_lane, v + 1);
             const double sm = lane_value(sq_lane, v);
-            f.x = fma(-cp, xp[k].y, f.x);
-            f.y = fma(cp, xp[k].x, f.y);
-            f = cfma(te[k].minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
+            f.x = fma(-cp, L[c].xp[k].y, f.x);
+            f.y = fma(cp, L[c].xp[k].x, f.y);
+            f = cfma(te[k].minus_c, make_double2(sm * L[c].xm[k].x, sm * L[c].xm[k].y), f);
         }
         stage_values<STAGE>(a.c1, s, f, ep, r0[c], r1[c]);
-#ifdef HEOM_PIPE_COLUMN_FENCE
-        __builtin_amdgcn_sched_barrier(0);  // columns do not interleave: bounded registers
-#endif
     }
 }
 
@@ -198,6 +217,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
     int t = chunk_tile(x, j, chunk);
     if (t >= nt)
         return;  // the whole workgroup leaves together
+    auto phys = [&](int tt) { return a.reverse ? nt - 1 - tt : tt; };
     const double sq_lane = sqrt((double)lane);
 
     double2 r0[kStageColumns], r1[kStageColumns];
@@ -205,7 +225,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
     for (int c = 0; c < kStageColumns; ++c)
         r0[c] = r1[c] = make_double2(0.0, 0.0);
 
-    prefetch_tile<N>(a, t, w, i, jb, lane, sigA, tabA);
+    prefetch_tile<N>(a, phys(t), w, i, jb, lane, sigA, tabA);
     __syncthreads();  // waits for the loads of this wave, then for every wave
 
     int prev = -1;
@@ -213,11 +233,12 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
         // tile t is in A; the next one goes to B
         j += gx;
         int tn = chunk_tile(x, j, chunk);
-        if (prev >= 0)
-            store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
-        if (tn < nt)
-            prefetch_tile<N>(a, tn, w, i, jb, lane, sigB, tabB);
-        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, t, i, jb, lane, sq_lane, sigA, tabA, r0, r1);
+        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigA, tabA, r0, r1, [&] {
+            if (prev >= 0)
+                store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+            if (tn < nt)
+                prefetch_tile<N>(a, phys(tn), w, i, jb, lane, sigB, tabB);
+        });
         prev = t;
         t = tn;
         __syncthreads();  // B is complete, and nobody reads A any more
@@ -227,17 +248,18 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
         // tile t is in B; the next one goes to A
         j += gx;
         tn = chunk_tile(x, j, chunk);
-        store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
-        if (tn < nt)
-            prefetch_tile<N>(a, tn, w, i, jb, lane, sigA, tabA);
-        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, t, i, jb, lane, sq_lane, sigB, tabB, r0, r1);
+        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigB, tabB, r0, r1, [&] {
+            store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+            if (tn < nt)
+                prefetch_tile<N>(a, phys(tn), w, i, jb, lane, sigA, tabA);
+        });
         prev = t;
         t = tn;
         __syncthreads();
         if (t >= nt)
             break;
     }
-    store_tile<N, STAGE>(a, prev, i, jb, lane, r0, r1);
+    store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
 }
 
 template <int N, int NT, bool REALH>
@@ -275,10 +297,12 @@ void launch_r(int N, int nt, int stage, const StageArgs& a, const ElementArgs& e
 
 }  // namespace
 
-bool pipe_supported(int N, int W)
+bool pipe_supported(int N, int W, int64_t mp)
 {
-    // wide enough workgroups to carry a CU on their own, and the table fits the LDS buffer
-    return N >= 6 && N <= kMaxStates && ((W + 2) & ~1) <= pipe_table_capacity(N);
+    // wide enough workgroups to carry a CU on their own, the table slice fits the LDS
+    // buffer, and table byte offsets fit in 32 bits
+    return N >= 6 && N <= kMaxStates && ((W + 2) & ~1) <= pipe_table_capacity(N) &&
+           (int64_t)(W + 2) * mp * (int64_t)sizeof(int2) < (int64_t(1) << 32);
 }
 
 void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
