@@ -1,13 +1,13 @@
 // heom_device.hpp -- device-side data layout, stage arguments and helpers shared by
 // the HIP translation units of libheom_hip.so.
 //
-// Device layout of one hierarchy (FP64):
+// Device layout of one hierarchy (FP64; FP32 storage halves the element size):
 //   * ADOs are renumbered into lexicographic order of their tuples (mode 0 most
 //     significant).  Neighbours n +- e_m then sit close in index: at 735k ADOs four
 //     fifths of all neighbour reads land within 30k ADOs, against one in twenty in the
 //     reference's layer order, so most of them are served from L2 / MALL.
 //   * State planes are structure-of-arrays by matrix element: Y[e * Mp + q] holds
-//     element e = i*N + j of ADO q as a double2 (re, im).  Mp = M + 1 rounded up to a
+//     element e = i*N + j of ADO q as a double2 (re, im), or a float2.  Mp = M + 1 rounded up to a
 //     multiple of 64.  Slot M (and every padding slot) is an ADO with no neighbours
 //     that stays exactly zero; edges that do not exist, or whose target is masked,
 //     point at slot M ("zero slot"), so gathers need no per-lane branches.
@@ -17,6 +17,11 @@
 //     row W serves the padding entries of the per-element term lists, and row W+1 lets
 //     the pipelined kernel stage rows in pairs.  Packing limits M to 2^26 - 1
 //     ADOs and n_m to 62.
+//
+//   * A batch of hierarchies with the same states and baths is one concatenated
+//     hierarchy: instance k owns the tiles from its tile offset on (its ADOs padded to
+//     a multiple of 64), its tables point into its own range, and a per-tile instance
+//     index selects its Hamiltonian.
 //
 // RK4 schedule (classical RK4, 12 state passes per step instead of 16): with
 // Y2 = y + h/2 k1, Y3 = y + h/2 k2, Y4 = y + h k3 and P = Y2 + 2 Y3 - y,
@@ -45,41 +50,73 @@ constexpr int kStageColumns = 4;    // matrix columns per wavefront of the stage
 // wavefronts per workgroup of the stage kernel: N rows x ceil(N / 4) column blocks
 __host__ __device__ constexpr int stage_waves(int N) { return N * ((N + kStageColumns - 1) / kStageColumns); }
 
+// Arithmetic and storage precision of the stage kernels: R = double (complex128
+// state) or R = float (complex64 state, FP32 arithmetic; the RK state of the tier-0/1
+// ADOs is kept in FP64 by a separate kernel, heom_hip.hip).
+template <typename R>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    using type = double2;
+};
+template <>
+struct Cplx<float> {
+    using type = float2;
+};
+template <typename R>
+using c2 = typename Cplx<R>::type;
+
+template <typename R>
+__host__ __device__ __forceinline__ c2<R> mk2(R x, R y)
+{
+    c2<R> v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
 // One neighbour term of one matrix element: mode m, and the constants of
 //   f_ij += i * plus_c * sqrt(n_m + 1) * sigma^{n+e_m}_ij + minus_c * sqrt(n_m) * sigma^{n-e_m}_ij
-struct Term {
-    double2 minus_c;
-    double plus_c;
+template <typename R>
+struct TermT {
+    c2<R> minus_c;
+    R plus_c;
     int32_t mode;
     int32_t info;  // bits 0-7: non-padding terms of this element (same in all its slots);
                    // bit 8: plus_c != 0; bit 9: minus_c != 0
 };
+using Term = TermT<double>;
 constexpr int32_t kTermHasPlus = 1 << 8;
 constexpr int32_t kTermHasMinus = 1 << 9;
 
-struct StageArgs {
-    const double2* __restrict__ yin;   // stage input Y_s (neighbours are read from here)
-    const double2* __restrict__ y0;    // y_n
-    double2* __restrict__ pbuf;        // P = Y2 + 2 Y3 - y
-    double2* __restrict__ yout;        // next stage input, y_{n+1}, or f (stage 0)
-    const int2* __restrict__ nb;       // [W+2][Mp] packed neighbour table
-    const double2* __restrict__ pfss;  // [Mp] pf_same_sum
+template <typename R>
+struct StageArgsT {
+    const c2<R>* __restrict__ yin;   // stage input Y_s (neighbours are read from here)
+    const c2<R>* __restrict__ y0;    // y_n
+    c2<R>* __restrict__ pbuf;        // P = Y2 + 2 Y3 - y
+    c2<R>* __restrict__ yout;        // next stage input, y_{n+1}, or f (stage 0)
+    const int2* __restrict__ nb;     // [W+2][Mp] packed neighbour table
+    const c2<R>* __restrict__ pfss;  // [Mp] pf_same_sum
+    const int32_t* __restrict__ tile_inst;  // [ntiles] instance of each tile, or null (one instance)
     int64_t mp;
     int32_t ntiles;       // Mp / 64
     int32_t zero_slot;    // index of the zero slot (M)
     int32_t table_units;  // (W + 2) / 2: pairs of table rows 0..W (pipelined kernel)
     int32_t xcd_chunk;    // tiles per XCD chunk (chunk_tile)
     int32_t reverse;      // walk the tiles from the last one (tile t -> ntiles - 1 - t)
-    double c1;            // stage coefficient (h/2, h/2, h, h/6)
+    R c1;                 // stage coefficient (h/2, h/2, h, h/6)
 };
+using StageArgs = StageArgsT<double>;
 
 // Element constants, read with wave-uniform indices; passed as separate restrict
 // kernel arguments so that the compiler can use scalar loads for them.
-struct ElementArgs {
-    const double2* hs;   // [N*N] H / hbar
-    const double2* gam;  // [N*N] -(Delta_b summed over the baths of i and of j), i != j
-    const Term* terms;   // [N*N][NT] neighbour terms per element (padded)
+template <typename R>
+struct ElementArgsT {
+    const c2<R>* hs;        // [instances][N*N] H / hbar
+    const c2<R>* gam;       // [N*N] -(Delta_b summed over the baths of i and of j), i != j
+    const TermT<R>* terms;  // [N*N][NT] neighbour terms per element (padded)
 };
+using ElementArgs = ElementArgsT<double>;
 
 // ---- helpers -------------------------------------------------------------------------
 
@@ -90,6 +127,21 @@ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c)  // c +
     c.y = fma(a.x, b.y, c.y);
     c.y = fma(a.y, b.x, c.y);
     return c;
+}
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c)
+{
+    c.x = fmaf(a.x, b.x, c.x);
+    c.x = fmaf(-a.y, b.y, c.x);
+    c.y = fmaf(a.x, b.y, c.y);
+    c.y = fmaf(a.y, b.x, c.y);
+    return c;
+}
+
+// H / hbar of the instance a tile belongs to
+template <typename R, int N>
+__device__ __forceinline__ const c2<R>* tile_hamiltonian(const c2<R>* hs, const int32_t* tile_inst, int tile)
+{
+    return tile_inst ? hs + (int64_t)tile_inst[tile] * (N * N) : hs;
 }
 
 // Tile order across XCDs.  Workgroups are dispatched round-robin over the XCDs
@@ -122,72 +174,79 @@ __device__ __forceinline__ double lane_value(double tab, int idx)
     const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(tab));
     return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ float lane_value(float tab, int idx)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(idx << 2, __float_as_int(tab)));
+}
 
 // RK4 stage combination for one element (see the schedule in the file header):
 // s = stage input, f = RHS at s, ep = y_n (stages 2, 3) or P (stage 4); r0 goes to the
 // stage output, r1 (stage 2 only) to P.
-template <int STAGE>
-__device__ __forceinline__ void stage_values(double c1, double2 s, double2 f, double2 ep, double2& r0, double2& r1)
+template <int STAGE, typename R>
+__device__ __forceinline__ void stage_values(R c1, c2<R> s, c2<R> f, c2<R> ep, c2<R>& r0, c2<R>& r1)
 {
     if (STAGE == 0) {
         r0 = f;
     } else if (STAGE == 1) {
-        r0 = make_double2(fma(c1, f.x, s.x), fma(c1, f.y, s.y));
+        r0 = mk2<R>(fma(c1, f.x, s.x), fma(c1, f.y, s.y));
     } else if (STAGE == 2) {
-        const double2 y3 = make_double2(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
+        const c2<R> y3 = mk2<R>(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
         r0 = y3;
-        r1 = make_double2(fma(2.0, y3.x, s.x) - ep.x, fma(2.0, y3.y, s.y) - ep.y);
+        r1 = mk2<R>(fma(R(2), y3.x, s.x) - ep.x, fma(R(2), y3.y, s.y) - ep.y);
     } else if (STAGE == 3) {
-        r0 = make_double2(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
+        r0 = mk2<R>(fma(c1, f.x, ep.x), fma(c1, f.y, ep.y));
     } else {
-        constexpr double third = 1.0 / 3.0;
-        r0 = make_double2(fma(ep.x + s.x, third, c1 * f.x), fma(ep.y + s.y, third, c1 * f.y));
+        constexpr R third = R(1) / R(3);
+        r0 = mk2<R>(fma(ep.x + s.x, third, c1 * f.x), fma(ep.y + s.y, third, c1 * f.y));
     }
 }
 
-// the epilogue operand of a stage: y_n for stages 2 and 3, P for stage 4
 // Streaming accesses with no reuse inside a stage (the stage operand y_n or P, and the
 // stage outputs) carry the non-temporal hint, so that they do not push out of the
 // caches the state lines that neighbour reads of other tiles will still ask for.
-typedef double heom_d2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 nt_load(const double2* p)
+template <typename R>
+using vec2_t = R __attribute__((ext_vector_type(2)));
+template <typename R>
+__device__ __forceinline__ c2<R> nt_load(const c2<R>* p)
 {
-    const heom_d2v v = __builtin_nontemporal_load((const heom_d2v*)p);
-    return make_double2(v.x, v.y);
+    const vec2_t<R> v = __builtin_nontemporal_load((const vec2_t<R>*)p);
+    return mk2<R>(v.x, v.y);
 }
-__device__ __forceinline__ void nt_store(double2 v, double2* p)
+template <typename R>
+__device__ __forceinline__ void nt_store(c2<R> v, c2<R>* p)
 {
-    heom_d2v w;
+    vec2_t<R> w;
     w.x = v.x;
     w.y = v.y;
-    __builtin_nontemporal_store(w, (heom_d2v*)p);
+    __builtin_nontemporal_store(w, (vec2_t<R>*)p);
 }
 
-template <int STAGE>
-__device__ __forceinline__ double2 stage_operand(const StageArgs& a, int64_t idx)
+// the epilogue operand of a stage: y_n for stages 2 and 3, P for stage 4
+template <int STAGE, typename R>
+__device__ __forceinline__ c2<R> stage_operand(const StageArgsT<R>& a, int64_t idx)
 {
     if (STAGE == 2 || STAGE == 3)
-        return nt_load(a.y0 + idx);
+        return nt_load<R>(a.y0 + idx);
     if (STAGE == 4)
-        return nt_load(a.pbuf + idx);
-    return make_double2(0.0, 0.0);
+        return nt_load<R>(a.pbuf + idx);
+    return mk2<R>(R(0), R(0));
 }
 
-template <int STAGE>
-__device__ __forceinline__ void stage_store(const StageArgs& a, int64_t idx, double2 r0, double2 r1)
+template <int STAGE, typename R>
+__device__ __forceinline__ void stage_store(const StageArgsT<R>& a, int64_t idx, c2<R> r0, c2<R> r1)
 {
-    nt_store(r0, a.yout + idx);
+    nt_store<R>(r0, a.yout + idx);
     if (STAGE == 2)
-        nt_store(r1, a.pbuf + idx);
+        nt_store<R>(r1, a.pbuf + idx);
 }
 
 // RK4 stage epilogue for one element
-template <int STAGE>
-__device__ __forceinline__ void stage_epilogue(const StageArgs& a, int64_t idx, double2 s, double2 f)
+template <int STAGE, typename R>
+__device__ __forceinline__ void stage_epilogue(const StageArgsT<R>& a, int64_t idx, c2<R> s, c2<R> f)
 {
-    double2 r0, r1 = make_double2(0.0, 0.0);
-    stage_values<STAGE>(a.c1, s, f, stage_operand<STAGE>(a, idx), r0, r1);
-    stage_store<STAGE>(a, idx, r0, r1);
+    c2<R> r0, r1 = mk2<R>(R(0), R(0));
+    stage_values<STAGE, R>(a.c1, s, f, stage_operand<STAGE, R>(a, idx), r0, r1);
+    stage_store<STAGE, R>(a, idx, r0, r1);
 }
 
 // ---- launcher (stage_kernel.hip) -------------------------------------------------------
@@ -196,22 +255,33 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs& a, int64_t idx, 
 bool stage_supported(int N, int nterms);
 // the padded term count the kernel for `nterms` expects
 int stage_term_slots(int nterms);
-void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a, const ElementArgs& el,
-                  hipStream_t s);
+void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                  const ElementArgsT<double>& el, hipStream_t s);
+void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                  const ElementArgsT<float>& el, hipStream_t s);
 
 // ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
 
+// LDS row stride (element slots per matrix row) of the pipelined kernel's tile: the
+// wavefronts' 4-column blocks, padded, so that 2-element loads never cross into a
+// slot of another wavefront
+__host__ __device__ constexpr int pipe_row_slots(int N) { return (N + kStageColumns - 1) / kStageColumns * kStageColumns; }
 // modes of the neighbour table one LDS buffer of the pipelined kernel holds: two state
-// tiles (N*N x 64 complex128) and two table tiles (capacity x 64 int2) in 160 KiB,
-// with 4 KiB to spare, an even number, at most 64
-__host__ __device__ constexpr int pipe_table_capacity(int N)
+// tiles (N x row slots x 64 complex) and two table tiles (capacity x 64 int2) in
+// 160 KiB, with 4 KiB to spare, an even number, at most 64
+__host__ __device__ constexpr int pipe_table_capacity(int N, int complex_bytes)
 {
-    return (((163840 - 2048 * N * N) / 1024 - 4) < 64 ? ((163840 - 2048 * N * N) / 1024 - 4) : 64) & ~1;
+    return (((163840 - 2 * N * pipe_row_slots(N) * 64 * complex_bytes) / 1024 - 4) < 64
+                ? ((163840 - 2 * N * pipe_row_slots(N) * 64 * complex_bytes) / 1024 - 4)
+                : 64) &
+           ~1;
 }
 // true if the pipelined kernel handles N states with W bath modes and Mp ADO slots
-bool pipe_supported(int N, int W, int64_t mp);
+bool pipe_supported(int N, int W, int64_t mp, int complex_bytes);
 // grid: `workgroups` persistent workgroups, a multiple of kNumXcd
-void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
-                            const ElementArgs& el, int workgroups, hipStream_t s);
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                            const ElementArgsT<double>& el, int workgroups, hipStream_t s);
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                            const ElementArgsT<float>& el, int workgroups, hipStream_t s);
 
 }  // namespace heom_hip

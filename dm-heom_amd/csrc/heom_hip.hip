@@ -1,6 +1,13 @@
 // heom_hip.hip -- the C ABI of include/heom_hip.h: handle lifetime, the re-layout of
-// the reference instance into the device layout (heom_device.hpp), the RK4 driver,
-// and the small layout / reduction / hierarchy-arithmetic kernels.
+// the reference instances into the device layout (heom_device.hpp), the RK4 driver,
+// the FP64 re-evaluation of the tier-0/1 ADOs of FP32-storage handles, and the small
+// layout / reduction / hierarchy-arithmetic kernels.
+//
+// A handle holds a batch of one or more hierarchies with the same states and baths,
+// concatenated into one device hierarchy (instance k owns the tiles from its tile
+// offset on); everything per instance (top ADO, Hamiltonian, mmult_left matrix,
+// recorded tops) is indexed by instance, everything per ADO (state, mask, norms) is the
+// instances' reference-layout arrays one after the other.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,24 +48,41 @@ void hip_check(hipError_t e, const char* what)
 #define HIPC(x) hip_check((x), #x)
 
 // ------------------------------------------------------------------------------------
-// layout, reduction and hierarchy-arithmetic kernels
+// layout, reduction and hierarchy-arithmetic kernels (C = double2 or float2 planes)
 // ------------------------------------------------------------------------------------
 
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+template <typename C>
+__device__ __forceinline__ C from_d2(double2 v);
+template <>
+__device__ __forceinline__ double2 from_d2<double2>(double2 v)
+{
+    return v;
+}
+template <>
+__device__ __forceinline__ float2 from_d2<float2>(double2 v)
+{
+    return make_float2((float)v.x, (float)v.y);
+}
+
 // dst[e * mp + q] = src[u(q)][e]   (reference AoS -> internal SoA; padding -> 0)
-__global__ void aos_to_soa(const double2* __restrict__ src, double2* __restrict__ dst,
-                           const int32_t* __restrict__ q2u, int64_t mp, int nel)
+template <typename C>
+__global__ void aos_to_soa(const double2* __restrict__ src, C* __restrict__ dst, const int32_t* __restrict__ q2u,
+                           int64_t mp, int nel)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= mp)
         return;
     const int32_t u = q2u[q];
     for (int e = 0; e < nel; ++e)
-        dst[(int64_t)e * mp + q] = u >= 0 ? src[(int64_t)u * nel + e] : make_double2(0.0, 0.0);
+        dst[(int64_t)e * mp + q] = from_d2<C>(u >= 0 ? src[(int64_t)u * nel + e] : make_double2(0.0, 0.0));
 }
 
 // dst[u(q)][e] = src[e * mp + q]   (internal SoA -> reference AoS)
-__global__ void soa_to_aos(const double2* __restrict__ src, double2* __restrict__ dst,
-                           const int32_t* __restrict__ q2u, int64_t mp, int nel)
+template <typename C>
+__global__ void soa_to_aos(const C* __restrict__ src, double2* __restrict__ dst, const int32_t* __restrict__ q2u,
+                           int64_t mp, int nel)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= mp)
@@ -67,30 +91,58 @@ __global__ void soa_to_aos(const double2* __restrict__ src, double2* __restrict_
     if (u < 0)
         return;
     for (int e = 0; e < nel; ++e)
-        dst[(int64_t)u * nel + e] = src[(int64_t)e * mp + q];
+        dst[(int64_t)u * nel + e] = to_d2(src[(int64_t)e * mp + q]);
 }
 
-// dst[e] = src[e * mp + q0] for one ADO (the top), e < nel
-__global__ void copy_ado_out(const double2* __restrict__ src, double2* __restrict__ dst, int64_t q0,
-                             int64_t mp, int nel)
+// dst[k][e] = src[e * mp + q[k]]: one ADO per instance (the tops), thread = (k, e)
+template <typename C>
+__global__ void gather_ados(const C* __restrict__ src, double2* __restrict__ dst, const int64_t* __restrict__ qs,
+                            int count, int64_t mp, int nel)
 {
-    const int e = threadIdx.x;
-    if (e < nel)
-        dst[e] = src[(int64_t)e * mp + q0];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel;
+    dst[t] = to_d2(src[(int64_t)e * mp + qs[k]]);
 }
 
-__global__ void copy_ado_in(const double2* __restrict__ src, double2* __restrict__ dst, int64_t q0,
-                            int64_t mp, int nel)
+template <typename C>
+__global__ void scatter_ados(const double2* __restrict__ src, C* __restrict__ dst, const int64_t* __restrict__ qs,
+                             int count, int64_t mp, int nel)
 {
-    const int e = threadIdx.x;
-    if (e < nel)
-        dst[(int64_t)e * mp + q0] = src[e];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel;
+    dst[(int64_t)e * mp + qs[k]] = from_d2<C>(src[t]);
+}
+
+// rows of the FP64 shadow: dst[k][e] = shadow[idx[k]][e]
+__global__ void gather_rows(const double2* __restrict__ shadow, double2* __restrict__ dst,
+                            const int32_t* __restrict__ idx, int count, int nel)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel;
+    dst[t] = shadow[(int64_t)idx[k] * nel + e];
+}
+
+__global__ void scatter_rows(const double2* __restrict__ src, double2* __restrict__ shadow,
+                             const int32_t* __restrict__ idx, int count, int nel)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel;
+    shadow[(int64_t)idx[k] * nel + e] = src[t];
 }
 
 // per-ADO squared Frobenius norm in uid order; threshold masking as in
 // dm-heom/cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
-__global__ void hierarchy_norm_kernel(const double2* __restrict__ y, const int32_t* __restrict__ q2u,
-                                      int64_t mp, int nel, double threshold, double* __restrict__ norms,
+template <typename C>
+__global__ void hierarchy_norm_kernel(const C* __restrict__ y, const int32_t* __restrict__ q2u, int64_t mp,
+                                      int nel, double threshold, double* __restrict__ norms,
                                       uint8_t* __restrict__ mask)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -101,7 +153,7 @@ __global__ void hierarchy_norm_kernel(const double2* __restrict__ y, const int32
         return;
     double acc = 0.0;
     for (int e = 0; e < nel; ++e) {
-        const double2 v = y[(int64_t)e * mp + q];
+        const double2 v = to_d2(y[(int64_t)e * mp + q]);
         acc = fma(v.x, v.x, acc);
         acc = fma(v.y, v.y, acc);
     }
@@ -126,18 +178,20 @@ __global__ void fold_mask(const int2* __restrict__ raw, int2* __restrict__ folde
     folded[t] = make_int2((int32_t)(((uint32_t)pf << 6) | (uint32_t)n), mf);
 }
 
-// sigma_q <- Mx sigma_q for every ADO: wave = row i, lane = ADO, tile in LDS
-template <int N>
-__global__ __launch_bounds__(64 * N) void mmult_left_kernel(double2* __restrict__ y, const double2* __restrict__ mx,
-                                                            int64_t mp)
+// sigma_q <- Mx_k sigma_q for every ADO q of instance k: wave = row i, lane = ADO,
+// tile in LDS, arithmetic in FP64
+template <typename C, int N>
+__global__ __launch_bounds__(64 * N) void mmult_left_kernel(C* __restrict__ y, const double2* __restrict__ mx,
+                                                            const int32_t* __restrict__ tile_inst, int64_t mp)
 {
     __shared__ double2 t[N * N * kTile];
     const int lane = threadIdx.x & 63;
     const int i = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * kTile + lane;
+    const double2* m_k = mx + (tile_inst ? (int64_t)tile_inst[blockIdx.x] * N * N : 0);
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        t[(i * N + j) * kTile + lane] = y[(int64_t)(i * N + j) * mp + q];
+        t[(i * N + j) * kTile + lane] = to_d2(y[(int64_t)(i * N + j) * mp + q]);
     __syncthreads();
     double2 out[N];
 #pragma unroll
@@ -145,14 +199,110 @@ __global__ __launch_bounds__(64 * N) void mmult_left_kernel(double2* __restrict_
         out[j] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        const double2 m = mx[i * N + k];
+        const double2 m = m_k[i * N + k];
 #pragma unroll
         for (int j = 0; j < N; ++j)
             out[j] = cfma(m, t[(k * N + j) * kTile + lane], out[j]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        y[(int64_t)(i * N + j) * mp + q] = out[j];
+        y[(int64_t)(i * N + j) * mp + q] = from_d2<C>(out[j]);
+}
+
+// mmult_left on the FP64 shadow rows: thread = (row r, element i*N+j)
+__global__ void mmult_left_rows(const double2* __restrict__ in, double2* __restrict__ out,
+                                const double2* __restrict__ mx, const int32_t* __restrict__ row_inst, int rows, int N)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nel = N * N;
+    if (t >= rows * nel)
+        return;
+    const int r = t / nel, e = t % nel, i = e / N, j = e % N;
+    const double2* m = mx + (int64_t)row_inst[r] * nel;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k)
+        acc = cfma(m[i * N + k], in[(int64_t)r * nel + k * N + j], acc);
+    out[t] = acc;
+}
+
+// ------------------------------------------------------------------------------------
+// FP32 storage: the tier-0/1 ADOs (tuple sum <= 1) in FP64
+// ------------------------------------------------------------------------------------
+//
+// With complex64 planes, the stage kernels evaluate the RHS in FP32 for every ADO.
+// For the top ADO and the W tier-1 ADOs this kernel then re-evaluates the RHS in FP64
+// and carries their RK4 state (y_n, the stage values and P) in FP64 shadows, writing
+// the rounded values into the planes for the neighbours to read.  Populations and
+// traces come from the FP64 top.  Thread = (tier-0/1 ADO s, element e).
+struct TierArgs {
+    const float2* yin;    // stage input planes (neighbours that are not tier-0/1)
+    float2* yout;         // stage output planes (overwritten at the tier-0/1 ADOs)
+    float2* pout;         // P planes (stage 2)
+    const int2* nb;       // folded neighbour table
+    const int64_t* q;     // [S] internal index
+    const int32_t* inst;  // [S] instance
+    const int2* sp_nb;    // [S][W] tier-0/1 index of the plus / minus neighbour, or -1
+    const double2* in64;  // [S][nel] stage input
+    double2* out64;       // [S][nel] stage output (stage 4: y_{n+1}, the y_n shadow itself)
+    const double2* y64;   // [S][nel] y_n
+    double2* p64;         // [S][nel] P
+    const double2* hs;    // [instances][nel] H / hbar
+    const double2* gam;   // [nel]
+    const Term* terms;    // [nel][nt]
+    const double2* pfss;  // [S] pf_same_sum
+    int64_t mp;
+    int32_t S, N, W, nt, zero;
+    double c1;
+};
+
+template <int STAGE>
+__global__ void tier_fixup(TierArgs a)
+{
+    const int nel = a.N * a.N;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.S * nel)
+        return;
+    const int s = t / nel, e = t % nel, i = e / a.N, j = e % a.N;
+    const int64_t q = a.q[s];
+    const double2* sig = a.in64 + (int64_t)s * nel;
+    const double2* H = a.hs + (int64_t)a.inst[s] * nel;
+    double2 cm = make_double2(0.0, 0.0);
+    for (int k = 0; k < a.N; ++k) {
+        cm = cfma(sig[i * a.N + k], H[k * a.N + j], cm);
+        const double2 hik = H[i * a.N + k];
+        cm = cfma(make_double2(-hik.x, -hik.y), sig[k * a.N + j], cm);
+    }
+    const double2 S = a.pfss[s], g = a.gam[e];
+    double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), sig[e], make_double2(-cm.y, cm.x));
+    for (int k = 0; k < a.nt; ++k) {
+        const Term tm = a.terms[e * a.nt + k];
+        if (tm.mode >= a.W)
+            continue;  // padding
+        const int2 v = a.nb[(int64_t)tm.mode * a.mp + q];
+        const int32_t p = (int32_t)((uint32_t)v.x >> 6), m = v.y, n = v.x & 63;
+        const int2 sn = a.sp_nb[(int64_t)s * a.W + tm.mode];
+        double2 xp = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+        if (p != a.zero)
+            xp = sn.x >= 0 ? a.in64[(int64_t)sn.x * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + p]);
+        if (m != a.zero)
+            xm = sn.y >= 0 ? a.in64[(int64_t)sn.y * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + m]);
+        const double cp = tm.plus_c * sqrt((double)(n + 1));
+        const double sm = sqrt((double)n);
+        f.x = fma(-cp, xp.y, f.x);
+        f.y = fma(cp, xp.x, f.y);
+        f = cfma(tm.minus_c, make_double2(sm * xm.x, sm * xm.y), f);
+    }
+    const int64_t row = (int64_t)s * nel + e;
+    const int64_t pl = (int64_t)e * a.mp + q;
+    double2 r0, r1 = make_double2(0.0, 0.0);
+    const double2 ep = STAGE == 4 ? a.p64[row] : (STAGE == 2 || STAGE == 3) ? a.y64[row] : make_double2(0.0, 0.0);
+    stage_values<STAGE, double>(a.c1, sig[e], f, ep, r0, r1);
+    a.out64[row] = r0;
+    a.yout[pl] = make_float2((float)r0.x, (float)r0.y);
+    if (STAGE == 2) {
+        a.p64[row] = r1;
+        a.pout[pl] = make_float2((float)r1.x, (float)r1.y);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -183,6 +333,11 @@ struct DevBuf {
         n = 0;
     }
     size_t bytes() const { return n * sizeof(T); }
+    template <typename U>
+    U* as() const
+    {
+        return reinterpret_cast<U*>(p);
+    }
 };
 
 }  // namespace
@@ -191,31 +346,49 @@ struct heom_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     int N = 0, K = 0, B = 0, BM = 0, W = 0, nel = 0;
-    int64_t M = 0, Mp = 0;
+    int prec = HEOM_PRECISION_FP64;
+    int count = 1;                  // instances in the batch
+    std::vector<int64_t> inst_M;    // ADOs per instance
+    std::vector<int64_t> inst_u;    // first global uid of each instance
+    std::vector<int64_t> inst_q;    // first internal index (a multiple of 64)
+    int64_t M = 0, Mp = 0;          // total ADOs (uid space), internal slots
     int ntiles = 0;
     int32_t zero_slot = 0;
     bool real_h = false;
-    int term_slots = 2;  // neighbour terms per element the stage kernel runs with
+    int term_slots = 2;      // neighbour terms per element the stage kernel runs with
     bool pipelined = false;  // stage_pipelined.hip (persistent) instead of stage_kernel.hip
     int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
     int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
-    int64_t q_top = 0;   // internal index of uid 0
 
     std::vector<int32_t> coupling;  // [N * BM]
     std::vector<double> cbk, pf_same, pf_circ;
 
-    DevBuf<double2> y, pbuf, ya, yb;
-    DevBuf<int2> nb_raw, nb;  // packed neighbour table, unmasked and mask-folded
-    DevBuf<int32_t> q2u;
+    DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
+    DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded
+    DevBuf<int32_t> q2u, tile_inst;
+    DevBuf<int64_t> q_top;                  // [count] internal index of each top ADO
     DevBuf<uint8_t> mask;
-    DevBuf<double2> pfss, hs, gam, mx, small;
+    DevBuf<double2> pfss, hs, gam, mx, small;  // FP64 constants
     DevBuf<Term> terms;
     DevBuf<double> norms;
+    // FP32 storage
+    DevBuf<float2> pfss32, hs32, gam32;
+    DevBuf<TermT<float>> terms32;
+    int S = 0;                    // tier-0/1 ADOs
+    std::vector<int32_t> sp_top;  // [count] tier-0/1 index of each top
+    std::vector<int64_t> sp_uid;  // [S] global uid
+    DevBuf<int64_t> sp_q;
+    DevBuf<int32_t> sp_inst, sp_top_d;
+    DevBuf<int2> sp_nb;
+    DevBuf<double2> y64, sa64, sb64, p64, sp_pfss, sp_tmp;
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timing_pending = false;
     int64_t pending_steps = 0;
     heom_timing_info timing{};
+
+    bool fp32() const { return prec == HEOM_PRECISION_FP32; }
+    size_t cbytes() const { return fp32() ? sizeof(float2) : sizeof(double2); }
 
     ~heom_handle()
     {
@@ -239,6 +412,8 @@ void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
     HIPC(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
 void resolve_timing(heom_handle* h)
 {
     if (!h->timing_pending)
@@ -253,28 +428,82 @@ void resolve_timing(heom_handle* h)
     h->pending_steps = 0;
 }
 
-void launch_stage(heom_handle* h, int stage, const double2* yin, double2* yout, double c1, bool reverse = false)
+template <typename R>
+void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigned char* yout, double c1, bool reverse,
+                    const ElementArgsT<R>& el, const c2<R>* pfss)
 {
-    StageArgs a{};
-    a.yin = yin;
-    a.y0 = h->y.p;
-    a.pbuf = h->pbuf.p;
-    a.yout = yout;
+    StageArgsT<R> a{};
+    a.yin = reinterpret_cast<const c2<R>*>(yin);
+    a.y0 = h->y.as<c2<R>>();
+    a.pbuf = h->pbuf.as<c2<R>>();
+    a.yout = reinterpret_cast<c2<R>*>(yout);
     a.nb = h->nb.p;
-    a.pfss = h->pfss.p;
+    a.pfss = pfss;
+    a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
     a.zero_slot = h->zero_slot;
     a.table_units = (h->W + 2) / 2;
     a.xcd_chunk = h->xcd_chunk;
     a.reverse = reverse ? 1 : 0;
-    a.c1 = c1;
-    ElementArgs el{h->hs.p, h->gam.p, h->terms.p};
+    a.c1 = (R)c1;
     if (h->pipelined)
         heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
     else
         heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
     HIPC(hipGetLastError());
+}
+
+// the FP64 pass over the tier-0/1 ADOs after a complex64 stage kernel
+void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsigned char* yout, double c1)
+{
+    TierArgs a{};
+    a.yin = reinterpret_cast<const float2*>(yin);
+    a.yout = reinterpret_cast<float2*>(yout);
+    a.pout = h->pbuf.as<float2>();
+    a.nb = h->nb.p;
+    a.q = h->sp_q.p;
+    a.inst = h->sp_inst.p;
+    a.sp_nb = h->sp_nb.p;
+    // shadows: stage 1 y -> A, stage 2 A -> B (+P), stage 3 B -> A, stage 4 A -> y;
+    // stage 0 (one RHS evaluation) y -> A
+    a.in64 = stage == 2 ? h->sa64.p : stage == 3 ? h->sb64.p : stage == 4 ? h->sa64.p : h->y64.p;
+    a.out64 = stage == 2 ? h->sb64.p : stage == 4 ? h->y64.p : h->sa64.p;
+    a.y64 = h->y64.p;
+    a.p64 = h->p64.p;
+    a.hs = h->hs.p;
+    a.gam = h->gam.p;
+    a.terms = h->terms.p;
+    a.pfss = h->sp_pfss.p;
+    a.mp = h->Mp;
+    a.S = h->S;
+    a.N = h->N;
+    a.W = h->W;
+    a.nt = h->term_slots;
+    a.zero = h->zero_slot;
+    a.c1 = c1;
+    const dim3 grid(blocks_for((int64_t)h->S * h->nel, 64)), block(64);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL(tier_fixup<0>, grid, block, 0, h->stream, a); break;
+    case 1: hipLaunchKernelGGL(tier_fixup<1>, grid, block, 0, h->stream, a); break;
+    case 2: hipLaunchKernelGGL(tier_fixup<2>, grid, block, 0, h->stream, a); break;
+    case 3: hipLaunchKernelGGL(tier_fixup<3>, grid, block, 0, h->stream, a); break;
+    default: hipLaunchKernelGGL(tier_fixup<4>, grid, block, 0, h->stream, a); break;
+    }
+    HIPC(hipGetLastError());
+}
+
+void launch_stage(heom_handle* h, int stage, const unsigned char* yin, unsigned char* yout, double c1,
+                  bool reverse = false)
+{
+    if (h->fp32()) {
+        const ElementArgsT<float> el{h->hs32.p, h->gam32.p, h->terms32.p};
+        launch_stage_r<float>(h, stage, yin, yout, c1, reverse, el, h->pfss32.p);
+        tier_fixup_launch(h, stage, yin, yout, c1);
+    } else {
+        const ElementArgsT<double> el{h->hs.p, h->gam.p, h->terms.p};
+        launch_stage_r<double>(h, stage, yin, yout, c1, reverse, el, h->pfss.p);
+    }
     h->timing.rhs_launches += 1;
 }
 
@@ -294,26 +523,32 @@ void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 void refold_tables(heom_handle* h)
 {
     const int64_t count = (int64_t)(h->W + 2) * h->Mp;
-    const int threads = 256;
-    const unsigned blocks = (unsigned)((count + threads - 1) / threads);
-    hipLaunchKernelGGL(fold_mask, dim3(blocks), dim3(threads), 0, h->stream, h->nb_raw.p, h->nb.p, h->mask.p,
-                       h->q2u.p, count, h->zero_slot);
+    hipLaunchKernelGGL(fold_mask, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->nb_raw.p, h->nb.p,
+                       h->mask.p, h->q2u.p, count, h->zero_slot);
     HIPC(hipGetLastError());
 }
 
-void set_hamiltonian(heom_handle* h, const double* H)
+// H [count][N*N*2] in J -> H / hbar on the device
+void set_hamiltonians(heom_handle* h, const double* H)
 {
-    const int N = h->N;
-    std::vector<double2> hs(N * N);
+    const size_t n = (size_t)h->count * h->nel;
+    std::vector<double2> hs(n);
+    std::vector<float2> hs32(n);
     bool real = true;
-    for (int e = 0; e < N * N; ++e) {
+    for (size_t e = 0; e < n; ++e) {
         hs[e] = make_double2(H[2 * e] / kHbar, H[2 * e + 1] / kHbar);
+        hs32[e] = make_float2((float)hs[e].x, (float)hs[e].y);
         if (H[2 * e + 1] != 0.0)
             real = false;
     }
     if (h->hs.p == nullptr)
-        h->hs.alloc(N * N);
-    HIPC(hipMemcpyAsync(h->hs.p, hs.data(), hs.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        h->hs.alloc(n);
+    HIPC(hipMemcpyAsync(h->hs.p, hs.data(), n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    if (h->fp32()) {
+        if (h->hs32.p == nullptr)
+            h->hs32.alloc(n);
+        HIPC(hipMemcpyAsync(h->hs32.p, hs32.data(), n * sizeof(float2), hipMemcpyHostToDevice, h->stream));
+    }
     HIPC(hipStreamSynchronize(h->stream));
     h->real_h = real;
 }
@@ -403,6 +638,20 @@ void build_coefficients(heom_handle* h)
         }
     upload(h->terms, flat, h->stream);
     upload(h->gam, gam, h->stream);
+    if (h->fp32()) {
+        std::vector<TermT<float>> f32(flat.size());
+        for (size_t t = 0; t < flat.size(); ++t) {
+            f32[t].minus_c = make_float2((float)flat[t].minus_c.x, (float)flat[t].minus_c.y);
+            f32[t].plus_c = (float)flat[t].plus_c;
+            f32[t].mode = flat[t].mode;
+            f32[t].info = flat[t].info;
+        }
+        std::vector<float2> g32(gam.size());
+        for (size_t e = 0; e < gam.size(); ++e)
+            g32[e] = make_float2((float)gam[e].x, (float)gam[e].y);
+        upload(h->terms32, f32, h->stream);
+        upload(h->gam32, g32, h->stream);
+    }
 }
 
 void validate(const heom_desc* d)
@@ -421,14 +670,32 @@ void validate(const heom_desc* d)
         raise(HEOM_E_INVALID, "matrices must be > 0");
     if (d->matrices > kMaxAdos - 2 * kTile)
         raise(HEOM_E_UNSUPPORTED, "more than 2^26 - 129 ADOs");
-    if (d->precision != HEOM_PRECISION_FP64)
-        raise(HEOM_E_UNSUPPORTED, "only HEOM_PRECISION_FP64 is supported by this build");
+    if (d->precision != HEOM_PRECISION_FP64 && d->precision != HEOM_PRECISION_FP32)
+        raise(HEOM_E_INVALID, "precision must be HEOM_PRECISION_FP64 or HEOM_PRECISION_FP32");
     if (!d->coupling || !d->hamiltonian || !d->ado_tuples || !d->ado_plus || !d->ado_minus || !d->pf_same_sum ||
         !d->pf_same || !d->pf_circ || !d->cbk)
         raise(HEOM_E_INVALID, "desc has a NULL array");
     for (int t = 0; t < d->states * d->baths_max_per_state; ++t)
         if (d->coupling[t] >= d->baths)
             raise(HEOM_E_INVALID, "coupling entry >= baths");
+}
+
+// the instances of a batch share states, baths, coupling and bath prefactors
+void validate_batch(const heom_desc* d, int count)
+{
+    for (int k = 1; k < count; ++k) {
+        const heom_desc& a = d[0];
+        const heom_desc& b = d[k];
+        if (a.states != b.states || a.matsubaras != b.matsubaras || a.baths != b.baths ||
+            a.baths_max_per_state != b.baths_max_per_state || a.width != b.width || a.precision != b.precision)
+            raise(HEOM_E_INVALID, "batch instances differ in states, baths, modes or precision");
+        const size_t nc = (size_t)a.states * a.baths_max_per_state;
+        if (!std::equal(a.coupling, a.coupling + nc, b.coupling) ||
+            !std::equal(a.pf_same, a.pf_same + 2 * a.baths, b.pf_same) ||
+            !std::equal(a.pf_circ, a.pf_circ + 2 * a.baths, b.pf_circ) ||
+            !std::equal(a.cbk, a.cbk + 2 * a.width, b.cbk))
+            raise(HEOM_E_INVALID, "batch instances differ in bath coupling or bath prefactors");
+    }
 }
 
 int guard(const std::function<void()>& fn)
@@ -465,7 +732,7 @@ void choose_stage_kernel(heom_handle* h)
 {
     const char* env = std::getenv("HEOM_STAGE_KERNEL");
     const bool want_tile = env && std::string(env) == "tile";
-    h->pipelined = !want_tile && pipe_supported(h->N, h->W, h->Mp);
+    h->pipelined = !want_tile && pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
     if (h->pipelined) {
         int cus = 0;
         HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -483,7 +750,261 @@ void choose_stage_kernel(heom_handle* h)
     }
 }
 
-unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+// the tier-0/1 ADOs of every instance, their FP64 shadows and neighbour map
+void setup_tiers(heom_handle* h, const heom_desc* d, const std::vector<std::vector<int32_t>>& u2q)
+{
+    const int W = h->W;
+    std::vector<int64_t> sq;
+    std::vector<int32_t> sinst;
+    std::vector<double2> spf;
+    std::vector<std::vector<int32_t>> local(h->count);  // local uid -> tier index, or -1
+    h->sp_top.assign(h->count, 0);
+    h->sp_uid.clear();
+    for (int k = 0; k < h->count; ++k) {
+        const int32_t* T = d[k].ado_tuples;
+        local[k].assign(d[k].matrices, -1);
+        for (int64_t u = 0; u < d[k].matrices; ++u) {
+            int64_t sum = 0;
+            for (int m = 0; m < W; ++m)
+                sum += T[u * W + m];
+            if (sum > 1)
+                continue;
+            local[k][u] = (int32_t)sq.size();
+            if (u == 0)
+                h->sp_top[k] = (int32_t)sq.size();
+            sq.push_back(h->inst_q[k] + u2q[k][u]);
+            sinst.push_back(k);
+            spf.push_back(make_double2(d[k].pf_same_sum[2 * u], d[k].pf_same_sum[2 * u + 1]));
+            h->sp_uid.push_back(h->inst_u[k] + u);
+        }
+    }
+    h->S = (int)sq.size();
+    std::vector<int2> snb((size_t)h->S * W, make_int2(-1, -1));
+    for (int s = 0; s < h->S; ++s) {
+        const int k = sinst[s];
+        const int64_t u = h->sp_uid[s] - h->inst_u[k];
+        for (int m = 0; m < W; ++m) {
+            const int32_t p = d[k].ado_plus[u * W + m], mn = d[k].ado_minus[u * W + m];
+            snb[(size_t)s * W + m] = make_int2(p >= 0 ? local[k][p] : -1, mn >= 0 ? local[k][mn] : -1);
+        }
+    }
+    upload(h->sp_q, sq, h->stream);
+    upload(h->sp_inst, sinst, h->stream);
+    upload(h->sp_pfss, spf, h->stream);
+    upload(h->sp_nb, snb, h->stream);
+    upload(h->sp_top_d, h->sp_top, h->stream);
+    for (auto* b : {&h->y64, &h->sa64, &h->sb64, &h->p64, &h->sp_tmp}) {
+        b->alloc((size_t)h->S * h->nel);
+        HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+    }
+}
+
+void create_batch(heom_handle* h, const heom_desc* d, int count)
+{
+    for (int k = 0; k < count; ++k)
+        validate(&d[k]);
+    validate_batch(d, count);
+    HIPC(hipGetDevice(&h->device));
+    HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPC(hipEventCreate(&h->ev_start));
+    HIPC(hipEventCreate(&h->ev_stop));
+    const int N = d[0].states, W = d[0].width;
+    h->N = N;
+    h->K = d[0].matsubaras;
+    h->B = d[0].baths;
+    h->BM = d[0].baths_max_per_state;
+    h->W = W;
+    h->nel = N * N;
+    h->prec = d[0].precision;
+    h->count = count;
+    h->coupling.assign(d[0].coupling, d[0].coupling + N * h->BM);
+    h->cbk.assign(d[0].cbk, d[0].cbk + 2 * h->B * h->K);
+    h->pf_same.assign(d[0].pf_same, d[0].pf_same + 2 * h->B);
+    h->pf_circ.assign(d[0].pf_circ, d[0].pf_circ + 2 * h->B);
+
+    // instance k: uids [inst_u, inst_u + M_k), internal slots from inst_q (a multiple of
+    // 64); the zero slot follows the last instance
+    h->inst_M.resize(count);
+    h->inst_u.resize(count);
+    h->inst_q.resize(count);
+    int64_t u0 = 0, q0 = 0;
+    for (int k = 0; k < count; ++k) {
+        h->inst_M[k] = d[k].matrices;
+        h->inst_u[k] = u0;
+        h->inst_q[k] = q0;
+        u0 += d[k].matrices;
+        q0 += k + 1 < count ? (d[k].matrices + kTile - 1) / kTile * kTile : d[k].matrices;
+    }
+    if (q0 + 1 > kMaxAdos - kTile)
+        raise(HEOM_E_UNSUPPORTED, "batch larger than 2^26 - 129 ADO slots");
+    h->M = u0;
+    h->zero_slot = (int32_t)q0;
+    h->Mp = (q0 + 1 + kTile - 1) / kTile * kTile;
+    h->ntiles = (int)(h->Mp / kTile);
+    const int64_t Mp = h->Mp;
+    const int32_t zero = h->zero_slot;
+
+    // rows 0..W-1 are the modes; rows W and W+1 hold the zero slot everywhere (row W
+    // serves padding terms, and the pipelined kernel loads rows in pairs)
+    std::vector<int2> nb((size_t)(W + 2) * Mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
+    std::vector<double2> pfss(Mp, make_double2(0.0, 0.0));
+    std::vector<int32_t> q2u(Mp, -1), tinst(h->ntiles, count - 1);
+    std::vector<int64_t> qtop(count);
+    std::vector<std::vector<int32_t>> u2q_all(count);
+    for (int k = 0; k < count; ++k) {
+        const int64_t M = d[k].matrices, qb = h->inst_q[k];
+        const int32_t* T = d[k].ado_tuples;
+        for (int64_t t = 0; t < M * W; ++t)
+            if (T[t] < 0 || T[t] > kMaxTupleValue)
+                raise(HEOM_E_UNSUPPORTED, "tuple entries must be in [0, 62]");
+        for (int m = 0; m < W; ++m)
+            if (T[m] != 0)
+                raise(HEOM_E_INVALID, "uid 0 must be the top ADO (all-zero tuple)");
+        // lexicographic ADO order (mode 0 most significant)
+        std::vector<int32_t> order(M);
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W, T + (int64_t)b * W,
+                                                T + (int64_t)b * W + W);
+        });
+        std::vector<int32_t>& u2q = u2q_all[k];
+        u2q.assign(M, 0);
+        for (int64_t q = 0; q < M; ++q) {
+            u2q[order[q]] = (int32_t)q;
+            q2u[qb + q] = (int32_t)(h->inst_u[k] + order[q]);
+        }
+        qtop[k] = qb + u2q[0];
+        for (int64_t t = qb / kTile; t < (qb + M + kTile - 1) / kTile; ++t)
+            tinst[t] = k;
+        for (int64_t q = 0; q < M; ++q) {
+            const int64_t u = order[q];
+            for (int m = 0; m < W; ++m) {
+                const int32_t p = d[k].ado_plus[u * W + m];
+                const int32_t mn = d[k].ado_minus[u * W + m];
+                if (p >= M || mn >= M)
+                    raise(HEOM_E_INVALID, "plus/minus entry out of range");
+                const int32_t pq = p >= 0 ? (int32_t)(qb + u2q[p]) : zero;
+                const int32_t mq = (mn >= 0 && T[u * W + m] > 0) ? (int32_t)(qb + u2q[mn]) : zero;
+                nb[(size_t)m * Mp + qb + q] = make_int2((int32_t)(((uint32_t)pq << 6) | (uint32_t)T[u * W + m]), mq);
+            }
+            pfss[qb + q] = make_double2(d[k].pf_same_sum[2 * u], d[k].pf_same_sum[2 * u + 1]);
+        }
+    }
+    upload(h->nb_raw, nb, h->stream);
+    upload(h->nb, nb, h->stream);
+    upload(h->pfss, pfss, h->stream);
+    upload(h->q2u, q2u, h->stream);
+    upload(h->tile_inst, tinst, h->stream);
+    upload(h->q_top, qtop, h->stream);
+    if (h->fp32()) {
+        std::vector<float2> p32(Mp);
+        for (int64_t q = 0; q < Mp; ++q)
+            p32[q] = make_float2((float)pfss[q].x, (float)pfss[q].y);
+        upload(h->pfss32, p32, h->stream);
+    }
+    h->mask.alloc(h->M);
+    HIPC(hipMemsetAsync(h->mask.p, 1, h->M, h->stream));
+    h->norms.alloc(h->M);
+    h->mx.alloc((size_t)count * h->nel);
+    h->small.alloc((size_t)count * h->nel);
+    const size_t plane = (size_t)N * N * Mp * h->cbytes();
+    for (auto* b : {&h->y, &h->pbuf, &h->ya, &h->yb}) {
+        b->alloc(plane);
+        HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+    }
+    build_coefficients(h);
+    std::vector<double> H((size_t)count * h->nel * 2);
+    for (int k = 0; k < count; ++k)
+        std::copy(d[k].hamiltonian, d[k].hamiltonian + 2 * h->nel, H.begin() + (size_t)k * h->nel * 2);
+    set_hamiltonians(h, H.data());
+    if (h->fp32())
+        setup_tiers(h, d, u2q_all);
+    choose_stage_kernel(h);
+    HIPC(hipStreamSynchronize(h->stream));
+}
+
+// state in reference layout (AoS, all instances) -> planes (and FP64 tier shadows)
+void upload_state(heom_handle* h, const double* state)
+{
+    DevBuf<double2> stage;
+    stage.alloc((size_t)h->M * h->nel);
+    HIPC(hipMemcpyAsync(stage.p, state, stage.bytes(), hipMemcpyHostToDevice, h->stream));
+    if (h->fp32()) {
+        hipLaunchKernelGGL(aos_to_soa<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, stage.p,
+                           h->y.as<float2>(), h->q2u.p, h->Mp, h->nel);
+        std::vector<double2> rows((size_t)h->S * h->nel);
+        for (int s = 0; s < h->S; ++s)
+            std::memcpy(&rows[(size_t)s * h->nel], state + (size_t)h->sp_uid[s] * h->nel * 2,
+                        h->nel * sizeof(double2));
+        HIPC(hipMemcpyAsync(h->y64.p, rows.data(), rows.size() * sizeof(double2), hipMemcpyHostToDevice,
+                            h->stream));
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(h->stream));
+    } else {
+        hipLaunchKernelGGL(aos_to_soa<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, stage.p,
+                           h->y.as<double2>(), h->q2u.p, h->Mp, h->nel);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(h->stream));
+    }
+}
+
+// planes `src` -> reference layout; with FP32 storage the tier-0/1 ADOs come from the
+// FP64 shadow `rows`
+void download_state(heom_handle* h, const unsigned char* src, const double2* rows, double* state)
+{
+    DevBuf<double2> stage;
+    stage.alloc((size_t)h->M * h->nel);
+    if (h->fp32())
+        hipLaunchKernelGGL(soa_to_aos<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
+                           reinterpret_cast<const float2*>(src), stage.p, h->q2u.p, h->Mp, h->nel);
+    else
+        hipLaunchKernelGGL(soa_to_aos<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
+                           reinterpret_cast<const double2*>(src), stage.p, h->q2u.p, h->Mp, h->nel);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(state, stage.p, stage.bytes(), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double2> sh;
+    if (h->fp32()) {
+        sh.resize((size_t)h->S * h->nel);
+        HIPC(hipMemcpyAsync(sh.data(), rows, sh.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPC(hipStreamSynchronize(h->stream));
+    if (h->fp32())
+        for (int s = 0; s < h->S; ++s)
+            std::memcpy(state + (size_t)h->sp_uid[s] * h->nel * 2, &sh[(size_t)s * h->nel],
+                        h->nel * sizeof(double2));
+}
+
+// the tops of all instances [count][nel] into dst (device)
+void gather_tops(heom_handle* h, double2* dst)
+{
+    const int n = h->count * h->nel;
+    if (h->fp32())
+        hipLaunchKernelGGL(gather_rows, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->y64.p, dst,
+                           h->sp_top_d.p, h->count, h->nel);
+    else
+        hipLaunchKernelGGL(gather_ados<double2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream,
+                           h->y.as<double2>(), dst, h->q_top.p, h->count, h->Mp, h->nel);
+    HIPC(hipGetLastError());
+}
+
+template <typename C>
+void mmult_left_planes(heom_handle* h)
+{
+    const dim3 grid(h->ntiles), block(64 * h->N);
+    C* y = h->y.as<C>();
+    const int32_t* ti = h->count > 1 ? h->tile_inst.p : nullptr;
+    switch (h->N) {
+    case 1: hipLaunchKernelGGL((mmult_left_kernel<C, 1>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 2: hipLaunchKernelGGL((mmult_left_kernel<C, 2>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 3: hipLaunchKernelGGL((mmult_left_kernel<C, 3>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 4: hipLaunchKernelGGL((mmult_left_kernel<C, 4>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 5: hipLaunchKernelGGL((mmult_left_kernel<C, 5>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 6: hipLaunchKernelGGL((mmult_left_kernel<C, 6>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 7: hipLaunchKernelGGL((mmult_left_kernel<C, 7>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    default: hipLaunchKernelGGL((mmult_left_kernel<C, 8>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    }
+    HIPC(hipGetLastError());
+}
 
 }  // namespace
 
@@ -493,7 +1014,7 @@ const char* heom_last_error(void) { return g_error.c_str(); }
 
 int heom_abi_version(void) { return HEOM_ABI_VERSION; }
 
-int heom_create(const heom_desc* d, heom_handle** out)
+int heom_create_batch(const heom_desc* descs, int32_t count, heom_handle** out)
 {
     if (!out) {
         g_error = "out is NULL";
@@ -502,87 +1023,12 @@ int heom_create(const heom_desc* d, heom_handle** out)
     *out = nullptr;
     heom_handle* h = nullptr;
     const int rc = guard([&] {
-        validate(d);
+        if (!descs)
+            raise(HEOM_E_INVALID, "desc is NULL");
+        if (count < 1)
+            raise(HEOM_E_INVALID, "count must be >= 1");
         h = new heom_handle();
-        HIPC(hipGetDevice(&h->device));
-        HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        HIPC(hipEventCreate(&h->ev_start));
-        HIPC(hipEventCreate(&h->ev_stop));
-        const int N = d->states, W = d->width;
-        const int64_t M = d->matrices;
-        h->N = N;
-        h->K = d->matsubaras;
-        h->B = d->baths;
-        h->BM = d->baths_max_per_state;
-        h->W = W;
-        h->nel = N * N;
-        h->M = M;
-        h->Mp = (M + 1 + kTile - 1) / kTile * kTile;  // at least one zero slot
-        h->ntiles = (int)(h->Mp / kTile);
-        h->zero_slot = (int32_t)M;
-        h->coupling.assign(d->coupling, d->coupling + N * h->BM);
-        h->cbk.assign(d->cbk, d->cbk + 2 * h->B * h->K);
-        h->pf_same.assign(d->pf_same, d->pf_same + 2 * h->B);
-        h->pf_circ.assign(d->pf_circ, d->pf_circ + 2 * h->B);
-
-        // lexicographic ADO order (mode 0 most significant)
-        const int32_t* T = d->ado_tuples;
-        for (int64_t t = 0; t < M * W; ++t)
-            if (T[t] < 0 || T[t] > kMaxTupleValue)
-                raise(HEOM_E_UNSUPPORTED, "tuple entries must be in [0, 62]");
-        std::vector<int32_t> order(M);
-        std::iota(order.begin(), order.end(), 0);
-        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W, T + (int64_t)b * W,
-                                                T + (int64_t)b * W + W);
-        });
-        const int64_t Mp = h->Mp;
-        std::vector<int32_t> q2u(Mp, -1), u2q(M);
-        for (int64_t q = 0; q < M; ++q) {
-            q2u[q] = order[q];
-            u2q[order[q]] = (int32_t)q;
-        }
-        for (int m = 0; m < W; ++m)
-            if (T[m] != 0)
-                raise(HEOM_E_INVALID, "uid 0 must be the top ADO (all-zero tuple)");
-        h->q_top = u2q[0];
-
-        const int32_t zero = h->zero_slot;
-        // rows 0..W-1 are the modes; rows W and W+1 hold the zero slot everywhere (row W
-        // serves padding terms, and the pipelined kernel loads rows in pairs)
-        std::vector<int2> nb((W + 2) * Mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
-        std::vector<double2> pfss(Mp, make_double2(0.0, 0.0));
-        for (int64_t q = 0; q < M; ++q) {
-            const int64_t u = order[q];
-            for (int m = 0; m < W; ++m) {
-                const int32_t p = d->ado_plus[u * W + m];
-                const int32_t mn = d->ado_minus[u * W + m];
-                if (p >= M || mn >= M)
-                    raise(HEOM_E_INVALID, "plus/minus entry out of range");
-                const int32_t pq = p >= 0 ? u2q[p] : zero;
-                const int32_t mq = (mn >= 0 && T[u * W + m] > 0) ? u2q[mn] : zero;
-                nb[m * Mp + q] = make_int2((int32_t)(((uint32_t)pq << 6) | (uint32_t)T[u * W + m]), mq);
-            }
-            pfss[q] = make_double2(d->pf_same_sum[2 * u], d->pf_same_sum[2 * u + 1]);
-        }
-        upload(h->nb_raw, nb, h->stream);
-        upload(h->nb, nb, h->stream);
-        upload(h->pfss, pfss, h->stream);
-        upload(h->q2u, q2u, h->stream);
-        h->mask.alloc(M);
-        HIPC(hipMemsetAsync(h->mask.p, 1, M, h->stream));
-        h->norms.alloc(M);
-        h->mx.alloc(N * N);
-        h->small.alloc(N * N);
-        const size_t plane = (size_t)N * N * Mp;
-        for (auto* b : {&h->y, &h->pbuf, &h->ya, &h->yb}) {
-            b->alloc(plane);
-            HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
-        }
-        build_coefficients(h);
-        set_hamiltonian(h, d->hamiltonian);
-        choose_stage_kernel(h);
-        HIPC(hipStreamSynchronize(h->stream));
+        create_batch(h, descs, count);
     });
     if (rc != HEOM_OK) {
         delete h;
@@ -592,11 +1038,23 @@ int heom_create(const heom_desc* d, heom_handle** out)
     return HEOM_OK;
 }
 
+int heom_create(const heom_desc* d, heom_handle** out) { return heom_create_batch(d, 1, out); }
+
 int heom_destroy(heom_handle* h)
 {
     return guard([&] {
         need(h);
         delete h;
+    });
+}
+
+int heom_instances(heom_handle* h, int32_t* count)
+{
+    return guard([&] {
+        need(h);
+        if (!count)
+            raise(HEOM_E_INVALID, "count is NULL");
+        *count = h->count;
     });
 }
 
@@ -606,7 +1064,7 @@ int heom_set_hamiltonian(heom_handle* h, const double* H)
         need(h);
         if (!H)
             raise(HEOM_E_INVALID, "hamiltonian is NULL");
-        set_hamiltonian(h, H);
+        set_hamiltonians(h, H);
     });
 }
 
@@ -642,12 +1100,7 @@ int heom_set_state(heom_handle* h, const double* state)
         need(h);
         if (!state)
             raise(HEOM_E_INVALID, "state is NULL");
-        const size_t bytes = (size_t)h->M * h->nel * sizeof(double2);
-        HIPC(hipMemcpyAsync(h->ya.p, state, bytes, hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(aos_to_soa, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->ya.p, h->y.p,
-                           h->q2u.p, h->Mp, h->nel);
-        HIPC(hipGetLastError());
-        HIPC(hipStreamSynchronize(h->stream));
+        upload_state(h, state);
     });
 }
 
@@ -657,9 +1110,17 @@ int heom_set_top(heom_handle* h, const double* top)
         need(h);
         if (!top)
             raise(HEOM_E_INVALID, "top is NULL");
-        HIPC(hipMemcpyAsync(h->small.p, top, h->nel * sizeof(double2), hipMemcpyHostToDevice, h->stream));
-        hipLaunchKernelGGL(copy_ado_in, dim3(1), dim3(((h->nel + 63) / 64) * 64), 0, h->stream, h->small.p, h->y.p,
-                           h->q_top, h->Mp, h->nel);
+        const int n = h->count * h->nel;
+        HIPC(hipMemcpyAsync(h->small.p, top, n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        if (h->fp32()) {
+            hipLaunchKernelGGL(scatter_ados<float2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
+                               h->y.as<float2>(), h->q_top.p, h->count, h->Mp, h->nel);
+            hipLaunchKernelGGL(scatter_rows, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
+                               h->y64.p, h->sp_top_d.p, h->count, h->nel);
+        } else {
+            hipLaunchKernelGGL(scatter_ados<double2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
+                               h->y.as<double2>(), h->q_top.p, h->count, h->Mp, h->nel);
+        }
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
     });
@@ -670,6 +1131,8 @@ int heom_clear_state(heom_handle* h)
     return guard([&] {
         need(h);
         HIPC(hipMemsetAsync(h->y.p, 0, h->y.bytes(), h->stream));
+        if (h->fp32())
+            HIPC(hipMemsetAsync(h->y64.p, 0, h->y64.bytes(), h->stream));
         HIPC(hipStreamSynchronize(h->stream));
     });
 }
@@ -680,12 +1143,7 @@ int heom_get_state(heom_handle* h, double* state)
         need(h);
         if (!state)
             raise(HEOM_E_INVALID, "state is NULL");
-        hipLaunchKernelGGL(soa_to_aos, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->y.p, h->yb.p,
-                           h->q2u.p, h->Mp, h->nel);
-        HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(state, h->yb.p, (size_t)h->M * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
-                            h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
+        download_state(h, h->y.p, h->y64.p, state);
     });
 }
 
@@ -695,10 +1153,9 @@ int heom_get_top(heom_handle* h, double* top)
         need(h);
         if (!top)
             raise(HEOM_E_INVALID, "top is NULL");
-        hipLaunchKernelGGL(copy_ado_out, dim3(1), dim3(((h->nel + 63) / 64) * 64), 0, h->stream, h->y.p,
-                           h->small.p, h->q_top, h->Mp, h->nel);
-        HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(top, h->small.p, h->nel * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        gather_tops(h, h->small.p);
+        HIPC(hipMemcpyAsync(top, h->small.p, (size_t)h->count * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
+                            h->stream));
         HIPC(hipStreamSynchronize(h->stream));
     });
 }
@@ -709,8 +1166,13 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
         need(h);
         if (!norms)
             raise(HEOM_E_INVALID, "norms is NULL");
-        hipLaunchKernelGGL(hierarchy_norm_kernel, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->y.p,
-                           h->q2u.p, h->Mp, h->nel, threshold, h->norms.p, h->mask.p);
+        if (h->fp32())
+            hipLaunchKernelGGL(hierarchy_norm_kernel<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
+                               h->y.as<float2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p, h->mask.p);
+        else
+            hipLaunchKernelGGL(hierarchy_norm_kernel<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0,
+                               h->stream, h->y.as<double2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p,
+                               h->mask.p);
         HIPC(hipGetLastError());
         if (threshold >= 0.0)
             refold_tables(h);
@@ -726,12 +1188,7 @@ int heom_rhs(heom_handle* h, double* out)
         if (!out)
             raise(HEOM_E_INVALID, "out is NULL");
         launch_stage(h, 0, h->y.p, h->ya.p, 0.0);
-        hipLaunchKernelGGL(soa_to_aos, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->ya.p, h->yb.p,
-                           h->q2u.p, h->Mp, h->nel);
-        HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(out, h->yb.p, (size_t)h->M * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
-                            h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
+        download_state(h, h->ya.p, h->sa64.p, out);
     });
 }
 
@@ -763,29 +1220,26 @@ int heom_propagate(heom_handle* h, double dt, int64_t nsteps, int64_t observe_st
         if (!(dt > 0.0))
             raise(HEOM_E_INVALID, "step size must be > 0");
         const int64_t rows = nsteps / observe_steps + 1;
+        const int64_t per = (int64_t)h->count * h->nel;
         DevBuf<double2> rec;
-        rec.alloc((size_t)rows * h->nel);
-        const int tb = ((h->nel + 63) / 64) * 64;
+        rec.alloc((size_t)(rows * per));
         resolve_timing(h);
         HIPC(hipEventRecord(h->ev_start, h->stream));
-        hipLaunchKernelGGL(copy_ado_out, dim3(1), dim3(tb), 0, h->stream, h->y.p, rec.p, h->q_top, h->Mp, h->nel);
+        gather_tops(h, rec.p);
         int64_t row = 1, done = 0;
         while (done < nsteps) {
             const int64_t n = std::min(observe_steps, nsteps - done);
             rk4_steps(h, dt, n);
             done += n;
             if (n == observe_steps) {
-                hipLaunchKernelGGL(copy_ado_out, dim3(1), dim3(tb), 0, h->stream, h->y.p, rec.p + row * h->nel,
-                                   h->q_top, h->Mp, h->nel);
+                gather_tops(h, rec.p + row * per);
                 ++row;
             }
         }
-        HIPC(hipGetLastError());
         HIPC(hipEventRecord(h->ev_stop, h->stream));
         h->timing_pending = true;
         h->pending_steps = nsteps;
-        HIPC(hipMemcpyAsync(tops, rec.p, (size_t)rows * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
-                            h->stream));
+        HIPC(hipMemcpyAsync(tops, rec.p, rec.bytes(), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         resolve_timing(h);
     });
@@ -797,17 +1251,16 @@ int heom_mmult_left(heom_handle* h, const double* mx)
         need(h);
         if (!mx)
             raise(HEOM_E_INVALID, "mx is NULL");
-        HIPC(hipMemcpyAsync(h->mx.p, mx, h->nel * sizeof(double2), hipMemcpyHostToDevice, h->stream));
-        const dim3 grid(h->ntiles), block(64 * h->N);
-        switch (h->N) {
-        case 1: hipLaunchKernelGGL(mmult_left_kernel<1>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 2: hipLaunchKernelGGL(mmult_left_kernel<2>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 3: hipLaunchKernelGGL(mmult_left_kernel<3>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 4: hipLaunchKernelGGL(mmult_left_kernel<4>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 5: hipLaunchKernelGGL(mmult_left_kernel<5>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 6: hipLaunchKernelGGL(mmult_left_kernel<6>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        case 7: hipLaunchKernelGGL(mmult_left_kernel<7>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
-        default: hipLaunchKernelGGL(mmult_left_kernel<8>, grid, block, 0, h->stream, h->y.p, h->mx.p, h->Mp); break;
+        const int n = h->count * h->nel;
+        HIPC(hipMemcpyAsync(h->mx.p, mx, n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        if (h->fp32()) {
+            mmult_left_planes<float2>(h);
+            const int m = h->S * h->nel;
+            hipLaunchKernelGGL(mmult_left_rows, dim3(blocks_for(m, 256)), dim3(256), 0, h->stream, h->y64.p,
+                               h->sp_tmp.p, h->mx.p, h->sp_inst.p, h->S, h->N);
+            HIPC(hipMemcpyAsync(h->y64.p, h->sp_tmp.p, h->y64.bytes(), hipMemcpyDeviceToDevice, h->stream));
+        } else {
+            mmult_left_planes<double2>(h);
         }
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
@@ -840,11 +1293,14 @@ int heom_device_bytes(heom_handle* h, int64_t* bytes)
         need(h);
         if (!bytes)
             raise(HEOM_E_INVALID, "bytes is NULL");
-        int64_t b = 0;
-        for (auto* x : {&h->y, &h->pbuf, &h->ya, &h->yb, &h->pfss, &h->hs, &h->gam, &h->mx, &h->small})
+        int64_t b = h->y.bytes() + h->pbuf.bytes() + h->ya.bytes() + h->yb.bytes();
+        for (auto* x : {&h->pfss, &h->hs, &h->gam, &h->mx, &h->small, &h->y64, &h->sa64, &h->sb64, &h->p64,
+                        &h->sp_pfss, &h->sp_tmp})
             b += x->bytes();
-        b += h->nb_raw.bytes() + h->nb.bytes() + h->q2u.bytes() + h->mask.bytes() + h->terms.bytes() +
-             h->norms.bytes();
+        b += h->nb_raw.bytes() + h->nb.bytes() + h->q2u.bytes() + h->tile_inst.bytes() + h->q_top.bytes() +
+             h->mask.bytes() + h->terms.bytes() + h->norms.bytes() + h->pfss32.bytes() + h->hs32.bytes() +
+             h->gam32.bytes() + h->terms32.bytes() + h->sp_q.bytes() + h->sp_inst.bytes() + h->sp_top_d.bytes() +
+             h->sp_nb.bytes();
         *bytes = b;
     });
 }
@@ -855,7 +1311,7 @@ int heom_rk4_step_bytes(heom_handle* h, int64_t* bytes)
         need(h);
         if (!bytes)
             raise(HEOM_E_INVALID, "bytes is NULL");
-        *bytes = (int64_t)16 * h->M * h->nel * (int64_t)sizeof(double2);
+        *bytes = (int64_t)16 * h->M * h->nel * (int64_t)h->cbytes();
     });
 }
 

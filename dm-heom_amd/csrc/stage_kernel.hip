@@ -21,21 +21,24 @@
 //   * the RK4 epilogue of heom_device.hpp.
 // Any coupling (several baths per state, shared baths, uncoupled states) runs through
 // the same code; the term loop is unrolled so a column's table loads and gathers are
-// all in flight together.
+// all in flight together.  The file is compiled twice: for complex128 state and FP64
+// arithmetic, and with HEOM_STAGE_PRECISION defined (stage_kernel_f32.hip) for complex64
+// state and FP32 arithmetic.
 #include "heom_device.hpp"
 
 namespace heom_hip {
 
 namespace {
 
-template <int N, int NT, int STAGE, bool REALH>
-__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, const double2* __restrict__ hs,
-                                                                  const double2* __restrict__ gam,
-                                                                  const Term* __restrict__ terms)
+template <typename R, int N, int NT, int STAGE, bool REALH>
+__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
+                                                                  const c2<R>* __restrict__ gam,
+                                                                  const TermT<R>* __restrict__ terms)
 {
+    using C = c2<R>;
     constexpr int CW = kStageColumns;
     constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
-    __shared__ double2 sig[N * N * kTile];
+    __shared__ C sig[N * N * kTile];
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -43,10 +46,11 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, co
     const int jb = (w % NJW) * CW;
     const int t0 = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
     const int tile = a.reverse ? a.ntiles - 1 - t0 : t0;
+    const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
 
-    double2 s[CW];
+    C s[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         const int j = jb + c;
@@ -54,17 +58,17 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, co
             s[c] = a.yin[(int64_t)(i * N + j) * mp + q];
             sig[(i * N + j) * kTile + lane] = s[c];
         } else {
-            s[c] = make_double2(0.0, 0.0);
+            s[c] = mk2<R>(R(0), R(0));
         }
     }
-    const double2 S = a.pfss[q];
+    const C S = a.pfss[q];
     __syncthreads();
 
-    double2 srow[N];
+    C srow[N];
 #pragma unroll
     for (int k = 0; k < N; ++k)
         srow[k] = sig[(i * N + k) * kTile + lane];
-    const double sq_lane = sqrt((double)lane);  // sqrt(n), n = 0..63, one value per lane
+    const R sq_lane = sqrt((R)lane);  // sqrt(n), n = 0..63, one value per lane
 
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
@@ -72,14 +76,14 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, co
         if (j >= N)
             break;
         const int e = i * N + j;
-        const double2* plane = a.yin + (int64_t)e * mp;
+        const C* plane = a.yin + (int64_t)e * mp;
 
         // neighbour terms of this element: tables and gathers first
         int2 nbv[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k)
             nbv[k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
-        double2 xp[NT], xm[NT];
+        C xp[NT], xm[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             xp[k] = plane[(uint32_t)nbv[k].x >> 6];
@@ -87,89 +91,106 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgs a, co
         }
 
         // commutator c = sum_k sigma_ik H_kj - H_ik sigma_kj
-        double2 cm = make_double2(0.0, 0.0);
+        C cm = mk2<R>(R(0), R(0));
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const double2 skj = sig[(k * N + j) * kTile + lane];
-            const double2 hkj = hs[k * N + j];
-            const double2 hik = hs[i * N + k];
+            const C skj = sig[(k * N + j) * kTile + lane];
+            const C hkj = hs[k * N + j];
+            const C hik = hs[i * N + k];
             if (REALH) {
                 cm.x = fma(srow[k].x, hkj.x, fma(-hik.x, skj.x, cm.x));
                 cm.y = fma(srow[k].y, hkj.x, fma(-hik.x, skj.y, cm.y));
             } else {
                 cm = cfma(srow[k], hkj, cm);
-                cm = cfma(make_double2(-hik.x, -hik.y), skj, cm);
+                cm = cfma(mk2<R>(-hik.x, -hik.y), skj, cm);
             }
         }
-        const double2 g = gam[e];
-        double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), s[c], make_double2(-cm.y, cm.x));
+        const C g = gam[e];
+        C f = cfma(mk2<R>(S.x + g.x, S.y + g.y), s[c], mk2<R>(-cm.y, cm.x));
 
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const Term t = terms[e * NT + k];
+            const TermT<R> t = terms[e * NT + k];
             const int v = nbv[k].x & 63;
-            const double cp = t.plus_c * lane_value(sq_lane, v + 1);
-            const double sm = lane_value(sq_lane, v);
+            const R cp = t.plus_c * lane_value(sq_lane, v + 1);
+            const R sm = lane_value(sq_lane, v);
             f.x = fma(-cp, xp[k].y, f.x);
             f.y = fma(cp, xp[k].x, f.y);
-            f = cfma(t.minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
+            f = cfma(t.minus_c, mk2<R>(sm * xm[k].x, sm * xm[k].y), f);
         }
 
-        stage_epilogue<STAGE>(a, (int64_t)e * mp + q, s[c], f);
+        stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, s[c], f);
     }
 }
 
-template <int N, int NT, bool REALH>
-void launch_t(int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+template <typename R, int N, int NT, bool REALH>
+void launch_t(int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     const dim3 grid(a.ntiles), block(64 * stage_waves(N));
     switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage<N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 1: hipLaunchKernelGGL((rk4_stage<N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 2: hipLaunchKernelGGL((rk4_stage<N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 3: hipLaunchKernelGGL((rk4_stage<N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    default: hipLaunchKernelGGL((rk4_stage<N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 0: hipLaunchKernelGGL((rk4_stage<R, N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage<R, N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage<R, N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage<R, N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage<R, N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
     }
 }
 
-template <int N, bool REALH>
-void launch_n(int nt, int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+template <typename R, int N, bool REALH>
+void launch_n(int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     switch (nt) {
-    case 2: launch_t<N, 2, REALH>(stage, a, el, st); break;
-    case 4: launch_t<N, 4, REALH>(stage, a, el, st); break;
-    default: launch_t<N, 8, REALH>(stage, a, el, st); break;
+    case 2: launch_t<R, N, 2, REALH>(stage, a, el, st); break;
+    case 4: launch_t<R, N, 4, REALH>(stage, a, el, st); break;
+    default: launch_t<R, N, 8, REALH>(stage, a, el, st); break;
     }
 }
 
-template <bool REALH>
-void launch_r(int N, int nt, int stage, const StageArgs& a, const ElementArgs& el, hipStream_t st)
+template <typename R, bool REALH>
+void launch_r(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     switch (N) {
-    case 1: launch_n<1, REALH>(nt, stage, a, el, st); break;
-    case 2: launch_n<2, REALH>(nt, stage, a, el, st); break;
-    case 3: launch_n<3, REALH>(nt, stage, a, el, st); break;
-    case 4: launch_n<4, REALH>(nt, stage, a, el, st); break;
-    case 5: launch_n<5, REALH>(nt, stage, a, el, st); break;
-    case 6: launch_n<6, REALH>(nt, stage, a, el, st); break;
-    case 7: launch_n<7, REALH>(nt, stage, a, el, st); break;
-    default: launch_n<8, REALH>(nt, stage, a, el, st); break;
+    case 1: launch_n<R, 1, REALH>(nt, stage, a, el, st); break;
+    case 2: launch_n<R, 2, REALH>(nt, stage, a, el, st); break;
+    case 3: launch_n<R, 3, REALH>(nt, stage, a, el, st); break;
+    case 4: launch_n<R, 4, REALH>(nt, stage, a, el, st); break;
+    case 5: launch_n<R, 5, REALH>(nt, stage, a, el, st); break;
+    case 6: launch_n<R, 6, REALH>(nt, stage, a, el, st); break;
+    case 7: launch_n<R, 7, REALH>(nt, stage, a, el, st); break;
+    default: launch_n<R, 8, REALH>(nt, stage, a, el, st); break;
     }
+}
+
+template <typename R>
+void launch_any(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el,
+                hipStream_t s)
+{
+    if (real_h)
+        launch_r<R, true>(N, nterm_slots, stage, a, el, s);
+    else
+        launch_r<R, false>(N, nterm_slots, stage, a, el, s);
 }
 
 }  // namespace
 
+#ifndef HEOM_STAGE_PRECISION
 int stage_term_slots(int nterms) { return nterms <= 2 ? 2 : nterms <= 4 ? 4 : 8; }
 
 bool stage_supported(int N, int nterms) { return N >= 1 && N <= kMaxStates && nterms >= 0 && nterms <= 8; }
+#endif
 
-void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a, const ElementArgs& el,
-                  hipStream_t s)
+#ifndef HEOM_STAGE_PRECISION
+void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                  const ElementArgsT<double>& el, hipStream_t s)
 {
-    if (real_h)
-        launch_r<true>(N, nterm_slots, stage, a, el, s);
-    else
-        launch_r<false>(N, nterm_slots, stage, a, el, s);
+    launch_any<double>(N, nterm_slots, real_h, stage, a, el, s);
 }
+#else
+void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                  const ElementArgsT<float>& el, hipStream_t s)
+{
+    launch_any<float>(N, nterm_slots, real_h, stage, a, el, s);
+}
+#endif
 
 }  // namespace heom_hip

@@ -17,6 +17,11 @@
 // The two buffers are separate LDS objects used with compile-time selection (the loop
 // is unrolled by two); that lets the compiler see that loads into one buffer do not
 // alias reads of the other, so it does not wait for the prefetch before computing.
+// LDS element slots are laid out per matrix row with the row padded to the wavefronts'
+// 4-column blocks (pipe_row_slots); with complex64 state one 16-byte-per-lane load
+// fills two adjacent slots, and the padding keeps it inside the loading wavefront's own
+// slots.  Compiled twice: complex128/FP64 here, complex64/FP32 with
+// HEOM_STAGE_PRECISION defined (stage_pipelined_f32.hip).
 #include "heom_device.hpp"
 
 namespace heom_hip {
@@ -28,30 +33,44 @@ namespace {
 // cache policy of the table loads: non-temporal (each table row is read once a stage)
 constexpr int kTableCachePolicy = 2;
 
-template <int N>
+template <typename R, int N>
 struct Shape {
     static constexpr int CW = kStageColumns;
     static constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
     static constexpr int NWAVE = N * NJW;
-    static constexpr int TC = pipe_table_capacity(N);
+    static constexpr int RS = pipe_row_slots(N);   // LDS element slots per matrix row
+    static constexpr int TC = pipe_table_capacity(N, (int)sizeof(c2<R>));
 };
 
 // issue the loads of tile `tile` into one LDS buffer: this wave's own elements and its
 // share of the table units (two modes x 64 ADOs per 16-byte-per-lane load)
-template <int N>
-__device__ __forceinline__ void prefetch_tile(const StageArgs& a, int tile, int w, int i, int jb, int lane,
-                                              double2* sig, int2* tab)
+template <typename R, int N>
+__device__ __forceinline__ void prefetch_tile(const StageArgsT<R>& a, int tile, int w, int i, int jb, int lane,
+                                              c2<R>* sig, int2* tab)
 {
-    using S = Shape<N>;
+    using S = Shape<R, N>;
     const int64_t mp = a.mp;
-    const uint32_t q = (uint32_t)tile * kTile + lane;
+    if (sizeof(c2<R>) == 16) {
+        const uint32_t q = (uint32_t)tile * kTile + lane;
 #pragma unroll
-    for (int c = 0; c < S::CW; ++c) {
-        const int j = jb + c;
-        if (j < N) {
-            const int e = i * N + j;
-            __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)e * mp + q), HEOM_LDS_PTR(sig + e * kTile),
-                                             16, 0, 0);
+        for (int c = 0; c < S::CW; ++c) {
+            const int j = jb + c;
+            if (j < N)
+                __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)(i * N + j) * mp + q),
+                                                 HEOM_LDS_PTR(sig + (i * S::RS + j) * kTile), 16, 0, 0);
+        }
+    } else {
+        // two elements per load: lanes 0-31 element j, lanes 32-63 element j+1 (j again
+        // when j+1 is past the row; it lands in a padding slot), two ADOs per lane
+        const uint32_t q2 = (uint32_t)tile * kTile + (lane & 31) * 2;
+#pragma unroll
+        for (int c = 0; c < S::CW; c += 2) {
+            const int j = jb + c;
+            if (j < N) {
+                const int jj = (lane >= 32 && j + 1 < N) ? j + 1 : j;
+                __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)(i * N + jj) * mp + q2),
+                                                 HEOM_LDS_PTR(sig + (i * S::RS + j) * kTile), 16, 0, 0);
+            }
         }
     }
     // table rows in pairs: lanes 0-31 load row 2u, lanes 32-63 row 2u+1, two ADOs per
@@ -74,13 +93,15 @@ __device__ __forceinline__ void prefetch_tile(const StageArgs& a, int tile, int 
 // after the first column's gathers are in flight: the previous tile's stores and the
 // next tile's prefetch go out there, behind those gathers, so that waiting for the
 // gathers does not mean waiting for the prefetch (loads complete in issue order).
-template <int N, int NT, int STAGE, bool REALH, typename Issue>
-__device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* __restrict__ hs,
-                                             const double2* __restrict__ gam, const Term* __restrict__ terms,
-                                             int tile, int i, int jb, int lane, double sq_lane, const double2* sig,
-                                             const int2* tab, double2 (&r0)[kStageColumns],
-                                             double2 (&r1)[kStageColumns], Issue&& issue)
+template <typename R, int N, int NT, int STAGE, bool REALH, typename Issue>
+__device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>* __restrict__ hs_all,
+                                             const c2<R>* __restrict__ gam, const TermT<R>* __restrict__ terms,
+                                             int tile, int i, int jb, int lane, R sq_lane, const c2<R>* sig,
+                                             const int2* tab, c2<R> (&r0)[kStageColumns],
+                                             c2<R> (&r1)[kStageColumns], Issue&& issue)
 {
+    using C = c2<R>;
+    using S = Shape<R, N>;
     constexpr int CW = kStageColumns;
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
@@ -92,7 +113,9 @@ __device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* 
         asm volatile("s_mov_b32 %0, 0" : "=s"(z));
         return p + z;
     };
-    const double2 S = a.pfss[q];
+    const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
+    const C S_ = a.pfss[q];
+    auto slot = [](int k, int j) { return (k * S::RS + j) * kTile; };
 
     // Column c's neighbour indices (from the LDS table; padding terms name mode W,
     // whose row is the zero slot) and its gathers (missing and masked neighbours read
@@ -101,29 +124,35 @@ __device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* 
     // issues its own at its start, and the commutator from LDS covers their latency.
     struct Loads {
         int2 nbv[NT];
-        double2 xp[NT], xm[NT];
+        C xp[NT], xm[NT];
     } L[CW];
+#ifdef HEOM_LDS_NEIGHBOURS
     // a neighbour inside this tile is read from the LDS copy of the tile; the others
     // from the element plane (in-tile lanes read the zero slot there, one cached line)
     const uint32_t zero = (uint32_t)a.zero_slot;
-    auto neighbour = [&](const double2* plane, const double2* tile_plane, uint32_t p) {
+    auto neighbour = [&](const C* plane, const C* tile_plane, uint32_t p) {
         const bool in = (p >> 6) == (uint32_t)tile;
-        const double2 g = plane[in ? zero : p];
-        const double2 l = tile_plane[p & 63];
+        const C g = plane[in ? zero : p];
+        const C l = tile_plane[p & 63];
         return in ? l : g;
     };
+#endif
+    // Columns past the row end (N not a multiple of 4) compute a copy of the last column
+    // and are not stored: every wavefront runs the same straight-line code, which keeps
+    // the per-column arrays in registers.
+    auto col = [&](int c) { return min(jb + c, N - 1); };
     auto load = [&](int c) {
-        const int e = i * N + jb + c;
-        const double2* plane = a.yin + (int64_t)e * mp;
-        const Term* te = opaque(terms) + e * NT;
+        const int e = i * N + col(c);
+        const C* plane = a.yin + (int64_t)e * mp;
+        const TermT<R>* te = opaque(terms) + e * NT;
 #pragma unroll
         for (int k = 0; k < NT; ++k)
             L[c].nbv[k] = tab[te[k].mode * kTile + lane];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
 #ifdef HEOM_LDS_NEIGHBOURS
-            L[c].xp[k] = neighbour(plane, sig + e * kTile, (uint32_t)L[c].nbv[k].x >> 6);
-            L[c].xm[k] = neighbour(plane, sig + e * kTile, (uint32_t)L[c].nbv[k].y);
+            L[c].xp[k] = neighbour(plane, sig + slot(i, col(c)), (uint32_t)L[c].nbv[k].x >> 6);
+            L[c].xm[k] = neighbour(plane, sig + slot(i, col(c)), (uint32_t)L[c].nbv[k].y);
 #else
             L[c].xp[k] = plane[(uint32_t)L[c].nbv[k].x >> 6];
             L[c].xm[k] = plane[(uint32_t)L[c].nbv[k].y];
@@ -138,68 +167,68 @@ __device__ __forceinline__ void compute_tile(const StageArgs& a, const double2* 
 
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
-        const int j = jb + c;
-        if (j >= N)
-            break;
+        const int j = col(c);
         const int e = i * N + j;
-        const double2 ep = stage_operand<STAGE>(a, (int64_t)e * mp + q);
+        const C ep = stage_operand<STAGE, R>(a, (int64_t)e * mp + q);
         if (c > 0)
             load(c);
-        const Term* te = opaque(terms) + e * NT;
-        const double2* hc = opaque(hs);
+        const TermT<R>* te = opaque(terms) + e * NT;
+        const C* hc = opaque(hs);
 
         // commutator c = sum_k sigma_ik H_kj - H_ik sigma_kj, row i and column j from the tile
-        double2 cm = make_double2(0.0, 0.0);
+        C cm = mk2<R>(R(0), R(0));
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const double2 sik = sig[(i * N + k) * kTile + lane];
-            const double2 skj = sig[(k * N + j) * kTile + lane];
-            const double2 hkj = hc[k * N + j];
-            const double2 hik = hc[i * N + k];
+            const C sik = sig[slot(i, k) + lane];
+            const C skj = sig[slot(k, j) + lane];
+            const C hkj = hc[k * N + j];
+            const C hik = hc[i * N + k];
             if (REALH) {
                 cm.x = fma(sik.x, hkj.x, fma(-hik.x, skj.x, cm.x));
                 cm.y = fma(sik.y, hkj.x, fma(-hik.x, skj.y, cm.y));
             } else {
                 cm = cfma(sik, hkj, cm);
-                cm = cfma(make_double2(-hik.x, -hik.y), skj, cm);
+                cm = cfma(mk2<R>(-hik.x, -hik.y), skj, cm);
             }
         }
-        const double2 s = sig[e * kTile + lane];
-        const double2 g = opaque(gam)[e];
-        double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), s, make_double2(-cm.y, cm.x));
+        const C s = sig[slot(i, j) + lane];
+        const C g = opaque(gam)[e];
+        C f = cfma(mk2<R>(S_.x + g.x, S_.y + g.y), s, mk2<R>(-cm.y, cm.x));
 
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int v = L[c].nbv[k].x & 63;
-            const double cp = te[k].plus_c * lane_value(sq_lane, v + 1);
-            const double sm = lane_value(sq_lane, v);
+            const R cp = te[k].plus_c * lane_value(sq_lane, v + 1);
+            const R sm = lane_value(sq_lane, v);
             f.x = fma(-cp, L[c].xp[k].y, f.x);
             f.y = fma(cp, L[c].xp[k].x, f.y);
-            f = cfma(te[k].minus_c, make_double2(sm * L[c].xm[k].x, sm * L[c].xm[k].y), f);
+            f = cfma(te[k].minus_c, mk2<R>(sm * L[c].xm[k].x, sm * L[c].xm[k].y), f);
         }
-        stage_values<STAGE>(a.c1, s, f, ep, r0[c], r1[c]);
+        stage_values<STAGE, R>(a.c1, s, f, ep, r0[c], r1[c]);
     }
 }
 
-template <int N, int STAGE>
-__device__ __forceinline__ void store_tile(const StageArgs& a, int tile, int i, int jb, int lane,
-                                           const double2 (&r0)[kStageColumns], const double2 (&r1)[kStageColumns])
+template <typename R, int N, int STAGE>
+__device__ __forceinline__ void store_tile(const StageArgsT<R>& a, int tile, int i, int jb, int lane,
+                                           const c2<R> (&r0)[kStageColumns], const c2<R> (&r1)[kStageColumns])
 {
     const uint32_t q = (uint32_t)tile * kTile + lane;
 #pragma unroll
     for (int c = 0; c < kStageColumns; ++c)
         if (jb + c < N)
-            stage_store<STAGE>(a, (int64_t)(i * N + jb + c) * a.mp + q, r0[c], r1[c]);
+            stage_store<STAGE, R>(a, (int64_t)(i * N + jb + c) * a.mp + q, r0[c], r1[c]);
 }
 
-template <int N, int NT, int STAGE, bool REALH>
-__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs a, const double2* __restrict__ hs,
-                                                                       const double2* __restrict__ gam,
-                                                                       const Term* __restrict__ terms)
+template <typename R, int N, int NT, int STAGE, bool REALH>
+__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgsT<R> a,
+                                                                       const c2<R>* __restrict__ hs,
+                                                                       const c2<R>* __restrict__ gam,
+                                                                       const TermT<R>* __restrict__ terms)
 {
-    using S = Shape<N>;
-    __shared__ __align__(16) double2 sigA[N * N * kTile];
-    __shared__ __align__(16) double2 sigB[N * N * kTile];
+    using S = Shape<R, N>;
+    using C = c2<R>;
+    __shared__ __align__(16) C sigA[N * S::RS * kTile];
+    __shared__ __align__(16) C sigB[N * S::RS * kTile];
     __shared__ __align__(16) int2 tabA[S::TC * kTile];
     __shared__ __align__(16) int2 tabB[S::TC * kTile];
 
@@ -218,14 +247,14 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
     if (t >= nt)
         return;  // the whole workgroup leaves together
     auto phys = [&](int tt) { return a.reverse ? nt - 1 - tt : tt; };
-    const double sq_lane = sqrt((double)lane);
+    const R sq_lane = sqrt((R)lane);
 
-    double2 r0[kStageColumns], r1[kStageColumns];
+    C r0[kStageColumns], r1[kStageColumns];
 #pragma unroll
     for (int c = 0; c < kStageColumns; ++c)
-        r0[c] = r1[c] = make_double2(0.0, 0.0);
+        r0[c] = r1[c] = mk2<R>(R(0), R(0));
 
-    prefetch_tile<N>(a, phys(t), w, i, jb, lane, sigA, tabA);
+    prefetch_tile<R, N>(a, phys(t), w, i, jb, lane, sigA, tabA);
     __syncthreads();  // waits for the loads of this wave, then for every wave
 
     int prev = -1;
@@ -233,12 +262,13 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
         // tile t is in A; the next one goes to B
         j += gx;
         int tn = chunk_tile(x, j, chunk);
-        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigA, tabA, r0, r1, [&] {
-            if (prev >= 0)
-                store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
-            if (tn < nt)
-                prefetch_tile<N>(a, phys(tn), w, i, jb, lane, sigB, tabB);
-        });
+        compute_tile<R, N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigA, tabA, r0, r1,
+                                             [&] {
+                                                 if (prev >= 0)
+                                                     store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+                                                 if (tn < nt)
+                                                     prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigB, tabB);
+                                             });
         prev = t;
         t = tn;
         __syncthreads();  // B is complete, and nobody reads A any more
@@ -248,70 +278,88 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgs 
         // tile t is in B; the next one goes to A
         j += gx;
         tn = chunk_tile(x, j, chunk);
-        compute_tile<N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigB, tabB, r0, r1, [&] {
-            store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
-            if (tn < nt)
-                prefetch_tile<N>(a, phys(tn), w, i, jb, lane, sigA, tabA);
-        });
+        compute_tile<R, N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigB, tabB, r0, r1,
+                                             [&] {
+                                                 store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+                                                 if (tn < nt)
+                                                     prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigA, tabA);
+                                             });
         prev = t;
         t = tn;
         __syncthreads();
         if (t >= nt)
             break;
     }
-    store_tile<N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+    store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
 }
 
-template <int N, int NT, bool REALH>
-void launch_t(int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+template <typename R, int N, int NT, bool REALH>
+void launch_t(int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int wgs, hipStream_t st)
 {
     const dim3 grid(wgs), block(64 * stage_waves(N));
     switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 1: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 2: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 3: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    default: hipLaunchKernelGGL((rk4_stage_pipe<N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 0: hipLaunchKernelGGL((rk4_stage_pipe<R, N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage_pipe<R, N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage_pipe<R, N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage_pipe<R, N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage_pipe<R, N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
     }
 }
 
-template <int N, bool REALH>
-void launch_n(int nt, int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+template <typename R, int N, bool REALH>
+void launch_n(int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int wgs, hipStream_t st)
 {
     switch (nt) {
-    case 2: launch_t<N, 2, REALH>(stage, a, el, wgs, st); break;
-    case 4: launch_t<N, 4, REALH>(stage, a, el, wgs, st); break;
-    default: launch_t<N, 8, REALH>(stage, a, el, wgs, st); break;
+    case 2: launch_t<R, N, 2, REALH>(stage, a, el, wgs, st); break;
+    case 4: launch_t<R, N, 4, REALH>(stage, a, el, wgs, st); break;
+    default: launch_t<R, N, 8, REALH>(stage, a, el, wgs, st); break;
     }
 }
 
-template <bool REALH>
-void launch_r(int N, int nt, int stage, const StageArgs& a, const ElementArgs& el, int wgs, hipStream_t st)
+template <typename R, bool REALH>
+void launch_r(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int wgs, hipStream_t st)
 {
     switch (N) {
-    case 6: launch_n<6, REALH>(nt, stage, a, el, wgs, st); break;
-    case 7: launch_n<7, REALH>(nt, stage, a, el, wgs, st); break;
-    default: launch_n<8, REALH>(nt, stage, a, el, wgs, st); break;
+    case 6: launch_n<R, 6, REALH>(nt, stage, a, el, wgs, st); break;
+    case 7: launch_n<R, 7, REALH>(nt, stage, a, el, wgs, st); break;
+    default: launch_n<R, 8, REALH>(nt, stage, a, el, wgs, st); break;
     }
+}
+
+template <typename R>
+void launch_any(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el,
+                int workgroups, hipStream_t s)
+{
+    if (real_h)
+        launch_r<R, true>(N, nterm_slots, stage, a, el, workgroups, s);
+    else
+        launch_r<R, false>(N, nterm_slots, stage, a, el, workgroups, s);
 }
 
 }  // namespace
 
-bool pipe_supported(int N, int W, int64_t mp)
+#ifndef HEOM_STAGE_PRECISION
+bool pipe_supported(int N, int W, int64_t mp, int complex_bytes)
 {
-    // wide enough workgroups to carry a CU on their own, the table slice fits the LDS
-    // buffer, and table byte offsets fit in 32 bits
-    return N >= 6 && N <= kMaxStates && ((W + 2) & ~1) <= pipe_table_capacity(N) &&
+    // wide enough workgroups to carry a CU on their own (with complex64 state only
+    // N = 8, whose column blocks are full: at N = 6, 7 that kernel spills registers), the
+    // table slice fits the LDS buffer, and table byte offsets fit in 32 bits
+    const bool shape = complex_bytes == (int)sizeof(double2) ? N >= 6 && N <= kMaxStates : N == kMaxStates;
+    return shape && ((W + 2) & ~1) <= pipe_table_capacity(N, complex_bytes) &&
            (int64_t)(W + 2) * mp * (int64_t)sizeof(int2) < (int64_t(1) << 32);
 }
 
-void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgs& a,
-                            const ElementArgs& el, int workgroups, hipStream_t s)
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                            const ElementArgsT<double>& el, int workgroups, hipStream_t s)
 {
-    if (real_h)
-        launch_r<true>(N, nterm_slots, stage, a, el, workgroups, s);
-    else
-        launch_r<false>(N, nterm_slots, stage, a, el, workgroups, s);
+    launch_any<double>(N, nterm_slots, real_h, stage, a, el, workgroups, s);
 }
+#else
+void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                            const ElementArgsT<float>& el, int workgroups, hipStream_t s)
+{
+    launch_any<float>(N, nterm_slots, real_h, stage, a, el, workgroups, s);
+}
+#endif
 
 }  // namespace heom_hip

@@ -28,10 +28,11 @@ HEOM_E_UNSUPPORTED = -2
 HEOM_E_DEVICE = -3
 HEOM_E_MEMORY = -4
 HEOM_PRECISION_FP64 = 0
+HEOM_PRECISION_FP32 = 1
 
 # every function include/heom_hip.h declares
 EXPORTED = [
-    "heom_create", "heom_destroy", "heom_last_error", "heom_abi_version",
+    "heom_create", "heom_create_batch", "heom_instances", "heom_destroy", "heom_last_error", "heom_abi_version",
     "heom_set_hamiltonian", "heom_set_mask", "heom_get_mask", "heom_set_state", "heom_set_top",
     "heom_clear_state",
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
@@ -85,6 +86,8 @@ def load():
         vp, i64, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double
         sig = {
             "heom_create": [ctypes.POINTER(Desc), ctypes.POINTER(vp)],
+            "heom_create_batch": [ctypes.POINTER(Desc), ctypes.c_int32, ctypes.POINTER(vp)],
+            "heom_instances": [vp, ctypes.POINTER(ctypes.c_int32)],
             "heom_destroy": [vp],
             "heom_abi_version": [],
             "heom_set_hamiltonian": [vp, vp],
@@ -130,66 +133,92 @@ def _c128_flat(m) -> np.ndarray:
     return _f64(np.stack([m.real, m.imag], axis=-1).reshape(-1))
 
 
+def _make_desc(states, matsubaras, baths, coupling, hamiltonian, tuples, plus, minus, pf_same_sum, pf_same,
+               pf_circ, cbk, precision=HEOM_PRECISION_FP64):
+    """A heom_desc over the reference instance arrays (returns the descriptor and the
+    arrays it points into, which must stay alive until heom_create returns)."""
+    coupling = np.ascontiguousarray(coupling, dtype=np.int32)
+    keep = dict(
+        coupling=coupling,
+        hamiltonian=_f64(hamiltonian).reshape(-1),
+        tuples=np.ascontiguousarray(tuples, dtype=np.int32).reshape(-1),
+        plus=np.ascontiguousarray(plus, dtype=np.int32).reshape(-1),
+        minus=np.ascontiguousarray(minus, dtype=np.int32).reshape(-1),
+        pf_same_sum=_f64(pf_same_sum), pf_same=_f64(pf_same), pf_circ=_f64(pf_circ), cbk=_f64(cbk),
+    )
+    width = int(baths) * int(matsubaras)
+    d = Desc()
+    d.states, d.matsubaras, d.baths = int(states), int(matsubaras), int(baths)
+    d.baths_max_per_state = int(coupling.shape[1]) if coupling.ndim == 2 else int(coupling.size // states)
+    d.width = width
+    d.precision = int(precision)
+    d.matrices = int(keep["tuples"].size // max(width, 1))
+    d.coupling = keep["coupling"].ctypes.data
+    d.hamiltonian = keep["hamiltonian"].ctypes.data
+    d.ado_tuples = keep["tuples"].ctypes.data
+    d.ado_plus = keep["plus"].ctypes.data
+    d.ado_minus = keep["minus"].ctypes.data
+    d.pf_same_sum = keep["pf_same_sum"].ctypes.data
+    d.pf_same = keep["pf_same"].ctypes.data
+    d.pf_circ = keep["pf_circ"].ctypes.data
+    d.cbk = keep["cbk"].ctypes.data
+    return d, keep
+
+
+def _instance_arrays(inst) -> dict:
+    return dict(states=inst.states, matsubaras=inst.matsubaras, baths=inst.baths,
+                coupling=np.asarray(inst.coupling).reshape(inst.states, -1), hamiltonian=inst.hamiltonian,
+                tuples=inst.tuples, plus=inst.plus, minus=inst.minus, pf_same_sum=inst.pf_same_sum,
+                pf_same=inst.pf_same, pf_circ=inst.pf_circ, cbk=inst.cbk)
+
+
 class Handle:
-    """One hierarchy on the GPU (owns all its device memory)."""
+    """One hierarchy, or a batch of hierarchies stepped together, on the GPU (owns all
+    its device memory).  With a batch (count > 1) per-instance arrays carry a leading
+    instance axis and per-ADO arrays are the instances' arrays one after the other."""
+
+    count = 1
 
     @classmethod
     def from_desc(cls, desc: Desc, keepalive=None) -> "Handle":
         """Create from a filled descriptor (e.g. HostInstance.desc())."""
+        return cls._create([desc], keepalive)
+
+    @classmethod
+    def _create(cls, descs, keepalive=None) -> "Handle":
         self = cls.__new__(cls)
+        arr = (Desc * len(descs))(*descs)
         h = ctypes.c_void_p()
-        _check(load().heom_create(ctypes.byref(desc), ctypes.byref(h)))
+        _check(load().heom_create_batch(arr, len(descs), ctypes.byref(h)))
         self._h = h
-        self.states = desc.states
-        self.matrices = desc.matrices
-        self.width = desc.width
+        self.count = len(descs)
+        self.states = descs[0].states
+        self.instance_matrices = [int(d.matrices) for d in descs]
+        self.matrices = sum(self.instance_matrices)
+        self.width = descs[0].width
+        self.precision = descs[0].precision
         self._keep = keepalive
         return self
 
     def __init__(self, *, states, matsubaras, baths, coupling, hamiltonian, tuples, plus, minus,
                  pf_same_sum, pf_same, pf_circ, cbk, precision=HEOM_PRECISION_FP64):
-        L = load()
-        coupling = np.ascontiguousarray(coupling, dtype=np.int32)
-        keep = dict(
-            coupling=coupling,
-            hamiltonian=_f64(hamiltonian).reshape(-1),
-            tuples=np.ascontiguousarray(tuples, dtype=np.int32).reshape(-1),
-            plus=np.ascontiguousarray(plus, dtype=np.int32).reshape(-1),
-            minus=np.ascontiguousarray(minus, dtype=np.int32).reshape(-1),
-            pf_same_sum=_f64(pf_same_sum), pf_same=_f64(pf_same), pf_circ=_f64(pf_circ), cbk=_f64(cbk),
-        )
-        width = int(baths) * int(matsubaras)
-        d = Desc()
-        d.states, d.matsubaras, d.baths = int(states), int(matsubaras), int(baths)
-        d.baths_max_per_state = int(coupling.shape[1]) if coupling.ndim == 2 else int(coupling.size // states)
-        d.width = width
-        d.precision = int(precision)
-        d.matrices = int(keep["tuples"].size // max(width, 1))
-        d.coupling = keep["coupling"].ctypes.data
-        d.hamiltonian = keep["hamiltonian"].ctypes.data
-        d.ado_tuples = keep["tuples"].ctypes.data
-        d.ado_plus = keep["plus"].ctypes.data
-        d.ado_minus = keep["minus"].ctypes.data
-        d.pf_same_sum = keep["pf_same_sum"].ctypes.data
-        d.pf_same = keep["pf_same"].ctypes.data
-        d.pf_circ = keep["pf_circ"].ctypes.data
-        d.cbk = keep["cbk"].ctypes.data
-        h = ctypes.c_void_p()
-        _check(L.heom_create(ctypes.byref(d), ctypes.byref(h)))
-        self._h = h
-        self.states = d.states
-        self.matrices = d.matrices
-        self.width = width
+        d, keep = _make_desc(states, matsubaras, baths, coupling, hamiltonian, tuples, plus, minus, pf_same_sum,
+                             pf_same, pf_circ, cbk, precision)
+        other = Handle._create([d], keep)
+        self.__dict__.update(other.__dict__)
+        other._h = None
 
     @classmethod
     def from_instance(cls, inst, precision=HEOM_PRECISION_FP64) -> "Handle":
         """Create from any object carrying the reference instance arrays (states,
         matsubaras, baths, coupling, hamiltonian, tuples, plus, minus, pf_*, cbk)."""
-        return cls(states=inst.states, matsubaras=inst.matsubaras, baths=inst.baths,
-                   coupling=np.asarray(inst.coupling).reshape(inst.states, -1),
-                   hamiltonian=inst.hamiltonian, tuples=inst.tuples, plus=inst.plus, minus=inst.minus,
-                   pf_same_sum=inst.pf_same_sum, pf_same=inst.pf_same, pf_circ=inst.pf_circ,
-                   cbk=inst.cbk, precision=precision)
+        return cls(precision=precision, **_instance_arrays(inst))
+
+    @classmethod
+    def from_instances(cls, insts, precision=HEOM_PRECISION_FP64) -> "Handle":
+        """A batch handle over several instances with the same states and baths."""
+        made = [_make_desc(precision=precision, **_instance_arrays(i)) for i in insts]
+        return cls._create([d for d, _ in made], [k for _, k in made])
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -220,15 +249,24 @@ class Handle:
             raise ValueError("state has the wrong size")
         _check(load().heom_set_state(self._h, s.ctypes.data))
 
+    def _per_instance(self, m) -> np.ndarray:
+        """[count][N][N] complex from one N x N matrix (used for every instance) or a stack."""
+        m = np.asarray(m)
+        if m.ndim == 2:
+            m = np.broadcast_to(m, (self.count,) + m.shape)
+        if m.shape != (self.count, self.states, self.states):
+            raise ValueError("expected an N x N matrix or one per instance")
+        return _c128_flat(m)
+
     def set_top(self, top) -> None:
-        t = _c128_flat(top)
+        t = self._per_instance(top)
         _check(load().heom_set_top(self._h, t.ctypes.data))
 
     def clear_state(self) -> None:
         _check(load().heom_clear_state(self._h))
 
     def set_hamiltonian(self, H) -> None:
-        h = _c128_flat(H) if np.iscomplexobj(H) else _f64(H)
+        h = self._per_instance(H)
         _check(load().heom_set_hamiltonian(self._h, h.ctypes.data))
 
     def set_mask(self, mask) -> None:
@@ -249,10 +287,12 @@ class Handle:
         return out
 
     def get_top(self) -> np.ndarray:
-        out = np.empty(self.states * self.states * 2)
+        """The top ADO, (N, N); for a batch (count, N, N)."""
+        out = np.empty(self.count * self.states * self.states * 2)
         _check(load().heom_get_top(self._h, out.ctypes.data))
-        t = out.reshape(self.states, self.states, 2)
-        return t[..., 0] + 1j * t[..., 1]
+        t = out.reshape(self.count, self.states, self.states, 2)
+        t = t[..., 0] + 1j * t[..., 1]
+        return t[0] if self.count == 1 else t
 
     def norms(self, threshold: float = -1.0) -> np.ndarray:
         out = np.empty(self.matrices)
@@ -269,14 +309,16 @@ class Handle:
         _check(load().heom_step_rk4(self._h, float(h), int(nsteps)))
 
     def propagate(self, h: float, nsteps: int, observe_steps: int) -> np.ndarray:
+        """Tops every observe_steps steps: (rows, N, N); for a batch (rows, count, N, N)."""
         rows = nsteps // observe_steps + 1
-        out = np.empty(rows * self.states * self.states * 2)
+        out = np.empty(rows * self.count * self.states * self.states * 2)
         _check(load().heom_propagate(self._h, float(h), int(nsteps), int(observe_steps), out.ctypes.data))
-        t = out.reshape(rows, self.states, self.states, 2)
-        return t[..., 0] + 1j * t[..., 1]
+        t = out.reshape(rows, self.count, self.states, self.states, 2)
+        t = t[..., 0] + 1j * t[..., 1]
+        return t[:, 0] if self.count == 1 else t
 
     def mmult_left(self, mx) -> None:
-        m = _c128_flat(mx)
+        m = self._per_instance(mx)
         _check(load().heom_mmult_left(self._h, m.ctypes.data))
 
     def sync(self) -> None:
@@ -462,8 +504,10 @@ class HostInstance:
         dm, dp = dm.reshape(n, n, 2), dp.reshape(n, n, 2)
         return dm[..., 0] + 1j * dm[..., 1], dp[..., 0] + 1j * dp[..., 1]
 
-    def create_handle(self) -> Handle:
+    def create_handle(self, precision: int = HEOM_PRECISION_FP64) -> Handle:
         """heom_create on the instance, with its initial top ADO loaded."""
-        h = Handle.from_desc(self.desc(), keepalive=self)
+        d = self.desc()
+        d.precision = int(precision)
+        h = Handle.from_desc(d, keepalive=self)
         h.set_top(self.top())
         return h

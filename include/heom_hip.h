@@ -45,7 +45,10 @@ enum heom_status {
 };
 
 enum heom_precision {
-    HEOM_PRECISION_FP64 = 0 /* complex128 ADO storage, FP64 RHS and RK4 */
+    HEOM_PRECISION_FP64 = 0, /* complex128 ADO storage, FP64 RHS and RK4 */
+    HEOM_PRECISION_FP32 = 1  /* complex64 ADO storage and FP32 RHS; the top ADO and the tier-1
+                                ADOs (tuple sum <= 1) are re-evaluated and accumulated in FP64,
+                                and tops, populations and traces are read from FP64 */
 };
 
 /* Static description of one hierarchy (the reference's heom::instance plus the
@@ -87,6 +90,17 @@ typedef struct heom_timing_info {
  * (dm-heom/include/heom/population_dynamics_solver.hpp:122-159).  Copies every array
  * of desc to the device.  The initial state is all zeros and the mask all TRUE. */
 int heom_create(const heom_desc* desc, heom_handle** out);
+/* A batch of `count` hierarchies in one handle (no counterpart in the reference, whose
+ * apps run one instance at a time; for ensembles such as static-disorder
+ * realisations).  The instances must share states, matsubaras, baths, coupling, the
+ * bath prefactors (pf_same, pf_circ, cbk) and precision; their Hamiltonians and
+ * depths may differ.  They are stepped together: one kernel launch per RK4 stage for
+ * the whole batch.  Per-instance arguments below ("[count]" arrays: top, hamiltonian,
+ * mmult_left matrices, recorded tops) are indexed by instance; per-ADO arrays (state,
+ * mask, norms, rhs) are the instances' arrays one after the other.  heom_create is
+ * heom_create_batch with count = 1. */
+int heom_create_batch(const heom_desc* descs, int32_t count, heom_handle** out);
+int heom_instances(heom_handle* handle, int32_t* count);
 int heom_destroy(heom_handle* handle);
 /* message of the last failing call on this thread ("" if none) */
 const char* heom_last_error(void);
@@ -95,7 +109,7 @@ int heom_abi_version(void);
 /* ---- inputs -------------------------------------------------------------------- */
 
 /* ode::update_hamiltonian (dm-heom/src/heom/ode.cpp:197-203) */
-int heom_set_hamiltonian(heom_handle* handle, const double* hamiltonian /* [N*N*2] */);
+int heom_set_hamiltonian(heom_handle* handle, const double* hamiltonian /* [count][N*N*2] */);
 /* solver_base::update_hierarchy_mask (dm-heom/include/heom/solver_base.hpp:292-299):
  * values 0 = FALSE, 1 = TRUE, 2 = PROTECTED; as in heom_ode.cl:199,237,297,360 a term
  * is dropped when the NEIGHBOUR it reads is masked. */
@@ -105,7 +119,7 @@ int heom_get_mask(heom_handle* handle, uint8_t* mask /* [M] */);
  * the whole state, reference layout. */
 int heom_set_state(heom_handle* handle, const double* state /* [M*N*N*2] */);
 /* solver_base::update_hierarchy_top (solver_base.hpp:283-289): overwrite ADO 0 only */
-int heom_set_top(heom_handle* handle, const double* top /* [N*N*2] */);
+int heom_set_top(heom_handle* handle, const double* top /* [count][N*N*2] */);
 /* instance::null_hierarchy (dm-heom/src/heom/instance.cpp:448-453) on the device:
  * every ADO zero (with heom_set_top: a fresh initial value without a host-side state) */
 int heom_clear_state(heom_handle* handle);
@@ -115,7 +129,7 @@ int heom_clear_state(heom_handle* handle);
 /* solver_base::hierarchy (solver_base.hpp:213-216) */
 int heom_get_state(heom_handle* handle, double* state /* [M*N*N*2] */);
 /* solver_base::hierarchy_top (solver_base.hpp:219-222) */
-int heom_get_top(heom_handle* handle, double* top /* [N*N*2] */);
+int heom_get_top(heom_handle* handle, double* top /* [count][N*N*2] */);
 /* solver_base::update_hierarchy_norm + hierarchy_norm (solver_base.hpp:224-228,
  * 270-273; kernel cl/heom/hierarchy_norm.cl:15-45): norms[n] = sum_ij |sigma_n,ij|^2 in
  * uid order; ADOs with norm - threshold <= 0 that are not PROTECTED are masked
@@ -138,10 +152,10 @@ int heom_step_rk4(heom_handle* handle, double h, int64_t nsteps);
  * synchronously after every step_forward(observe_steps) (app_population_dynamics.cpp:
  * 92-106); this does the same without a host round trip per observation. */
 int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
-                   double* tops /* [(nsteps/observe_steps + 1) * N*N*2] */);
+                   double* tops /* [nsteps/observe_steps + 1][count][N*N*2] */);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
  * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */
-int heom_mmult_left(heom_handle* handle, const double* mx /* [N*N*2] */);
+int heom_mmult_left(heom_handle* handle, const double* mx /* [count][N*N*2] */);
 /* wait for all work queued on the handle */
 int heom_sync(heom_handle* handle);
 int heom_timing(heom_handle* handle, heom_timing_info* info);
@@ -151,7 +165,7 @@ int heom_timing(heom_handle* handle, heom_timing_info* info);
 /* bytes of device memory held by the handle */
 int heom_device_bytes(heom_handle* handle, int64_t* bytes);
 /* algorithmic HBM bytes of one RK4 step of the whole hierarchy
- * (16 state passes x M x N^2 x sizeof(complex)) */
+ * (16 state passes x M x N^2 x sizeof(complex), complex128 or complex64) */
 int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
 /* which stage kernel the handle runs (no counterpart in the reference; for tests and
  * benchmarks): HEOM_STAGE_TILE = one workgroup per 64-ADO tile, HEOM_STAGE_PIPELINED =

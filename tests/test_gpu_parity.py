@@ -45,12 +45,15 @@ def make_instance(N, B, K, D, coupling, seed=0, complex_h=False, omega=None, tem
                       temperature=temperature)
 
 
-def expected_kernel(inst, requested):
-    """The pipelined kernel holds two N*N x 64 state tiles and two (W+1)-row table tiles
-    (rounded to an even row count) in 160 KiB of LDS with 4 KiB spare."""
+def expected_kernel(inst, requested, complex_bytes=16):
+    """The pipelined kernel holds two state tiles (N rows of ceil(N/4)*4 element slots x
+    64 ADOs) and two (W+2)-row table tiles (rounded to an even row count) in 160 KiB of
+    LDS with 4 KiB spare; with complex64 state it runs at N = 8 only."""
     N, W = inst.states, inst.width
-    cap = min(64, (163840 - 2048 * N * N) // 1024 - 4) & ~1
-    ok = requested == "pipelined" and N >= 6 and ((W + 2) & ~1) <= cap
+    rs = (N + 3) // 4 * 4
+    cap = min(64, (163840 - 2 * N * rs * 64 * complex_bytes) // 1024 - 4) & ~1
+    shape = N >= 6 if complex_bytes == 16 else N == 8
+    ok = requested == "pipelined" and shape and ((W + 2) & ~1) <= cap
     return "pipelined" if ok else "tile"
 
 
