@@ -1,6 +1,9 @@
 #!/usr/bin/env python3
 """Throughput of the HEOM hot path on one MI355X: classical RK4 over the full ADO
 hierarchy of the 8-site, 16-mode, depth-8 system (735,471 complex128 8x8 ADOs).
+Beside it: the same workload with FP32 storage (and its population error against FP64
+over 1000 fs), and the batched 512-realisation static-disorder ensemble in FP64 and
+FP32 (ensemble ADO-steps/s and the FP32 population error).
 
 One "step" is one RK4 step of the whole hierarchy (four fused RHS+RK stage kernels).
 The instance is built by the product's host path (config file -> C++ hierarchy graph
@@ -18,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dm-heom_amd"))
 
@@ -32,6 +37,9 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--config", default="excitonic8_d8")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the FP32-storage run of the same workload and the batched ensemble")
+    p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel of "
@@ -68,6 +76,54 @@ def cpu_baseline(cfg_path: str, target_s: float) -> dict:
         "sample": f"{n} RK4 steps of the same {inst.matrices}-ADO hierarchy after 1 warm-up step "
                   f"(unfused: 4 RHS + 4 weighted adds per step), {dt:.1f} s",
     }
+
+
+def precision_run(host, steps_1000fs: int) -> dict:
+    """The same workload with complex64 storage and FP32 RHS (FP64 tier-0/1 ADOs):
+    throughput, and the population error against FP64 over 1000 fs."""
+    import heom_amd
+
+    info = host.info
+    res = {}
+    pops = {}
+    for prec, name in ((heom_amd.HEOM_PRECISION_FP64, "fp64"), (heom_amd.HEOM_PRECISION_FP32, "fp32")):
+        h = host.create_handle(precision=prec)
+        p, _ = h.propagate_observe(info.step_size, steps_1000fs, 10)
+        t = h.timing().last_call_seconds
+        pops[name] = p[:, 0]
+        res[name] = {"ms_per_step": t / steps_1000fs * 1e3, "value": host.matrices * steps_1000fs / t}
+        h.close()
+    res["max_population_error_vs_fp64"] = float(np.max(np.abs(pops["fp32"] - pops["fp64"])))
+    res["span_fs"] = steps_1000fs * info.step_size * 1e15
+    res["unit"] = "ADO-RK4-steps/s"
+    return res
+
+
+def ensemble_run(members: int) -> dict:
+    """The 512-realisation static-disorder ensemble (7 sites, depths 3-6, populations
+    plus xx/yy/zz linear absorption, 2048 steps of 2 fs), batched, FP64 and FP32."""
+    import heom_amd
+    from heom_amd.ensemble import Ensemble
+
+    out = {}
+    results = {}
+    for prec, name in ((heom_amd.HEOM_PRECISION_FP64, "fp64"), (heom_amd.HEOM_PRECISION_FP32, "fp32")):
+        ens = Ensemble(members=members, precision=prec)
+        r = ens.run()
+        results[name] = r
+        out[name] = {"value": r["ado_steps"] / r["device_seconds"], "device_seconds": r["device_seconds"],
+                     "ms_per_step": r["device_seconds"] / r["steps"] * 1e3}
+        out["hierarchies"] = ens.members * (1 + ens.ncomp)
+        out["ados"] = r["ados"]
+        ens.close()
+    out["max_population_error_fp32_vs_fp64"] = float(
+        np.max(np.abs(results["fp32"]["populations"] - results["fp64"]["populations"])))
+    a, b = results["fp64"]["absorption"], results["fp32"]["absorption"]
+    out["max_absorption_error_fp32_vs_fp64"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+    out["unit"] = "ensemble ADO-RK4-steps/s"
+    out["workload"] = (f"{members} static-disorder realisations of the 7-site system, depths 3-6, populations + "
+                       f"linear absorption (3 tensor components, rotating frame), 2048 RK4 steps of 2 fs")
+    return out
 
 
 def main():
@@ -124,7 +180,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": "ADO-RK4-steps/s (+ % HBM peak); plain FP64 RK4 over the full hierarchy",
+        "metric": "ADO\u00b7RK4-steps/s + % HBM peak; batched FP32 vs plain FP64 at \u22641e-6 pop err",
         "value": value,
         "unit": "ADO-RK4-steps/s",
         "n_gpus": 1,
@@ -159,6 +215,10 @@ def main():
         "checks": {"top_trace_minus_1": trace_err},
     }
     h.close()
+    if not args.no_extras:
+        out["precision"] = precision_run(host, int(round(1e-12 / dt)))
+        out["ensemble"] = ensemble_run(args.ensemble_members)
+        out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value
     host.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg_path, args.cpu_seconds)

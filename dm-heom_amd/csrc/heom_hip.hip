@@ -138,6 +138,31 @@ __global__ void scatter_rows(const double2* __restrict__ src, double2* __restric
     shadow[(int64_t)idx[k] * nel + e] = src[t];
 }
 
+// Populations Re sigma_0[i][i] (dm-heom/include/heom/matrix_diagonal_observer.hpp:28-34)
+// and, with ops, traces Tr(ops_k sigma_0) (matrix_trace_observer.hpp:139-158) of every
+// instance's top ADO: one wavefront per instance, lane = element, a shuffle reduction
+// over the N*N products.
+__global__ __launch_bounds__(64) void observe_tops(const double2* __restrict__ tops, const double2* __restrict__ ops,
+                                                   double* __restrict__ pops, double2* __restrict__ traces, int N)
+{
+    const int k = blockIdx.x, lane = threadIdx.x, nel = N * N;
+    const int i = lane / N, j = lane % N;
+    const double2 s = lane < nel ? tops[(int64_t)k * nel + lane] : make_double2(0.0, 0.0);
+    if (lane < nel && i == j)
+        pops[(int64_t)k * N + i] = s.x;
+    if (ops) {
+        // Tr(D sigma) = sum_ij D_ji sigma_ij
+        const double2 d = lane < nel ? ops[(int64_t)k * nel + j * N + i] : make_double2(0.0, 0.0);
+        double re = fma(d.x, s.x, -d.y * s.y), im = fma(d.x, s.y, d.y * s.x);
+        for (int off = 32; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off);
+            im += __shfl_xor(im, off);
+        }
+        if (lane == 0)
+            traces[k] = make_double2(re, im);
+    }
+}
+
 // per-ADO squared Frobenius norm in uid order; threshold masking as in
 // dm-heom/cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
 template <typename C>
@@ -382,6 +407,12 @@ struct heom_handle {
     DevBuf<int2> sp_nb;
     DevBuf<double2> y64, sa64, sb64, p64, sp_pfss, sp_tmp;
 
+    // one RK4 step captured as a HIP graph (heom_use_graphs), replayed per step
+    bool use_graphs = false;
+    hipGraphExec_t step_graph = nullptr;
+    double graph_dt = 0.0;
+    bool graph_real_h = false;
+
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timing_pending = false;
     int64_t pending_steps = 0;
@@ -394,6 +425,8 @@ struct heom_handle {
     {
         if (stream)
             (void)hipStreamSynchronize(stream);
+        if (step_graph)
+            (void)hipGraphExecDestroy(step_graph);
         if (ev_start)
             (void)hipEventDestroy(ev_start);
         if (ev_stop)
@@ -507,17 +540,60 @@ void launch_stage(heom_handle* h, int stage, const unsigned char* yin, unsigned 
     h->timing.rhs_launches += 1;
 }
 
-// classical RK4 in the 12-pass schedule of heom_device.hpp
+// one classical RK4 step in the 12-pass schedule of heom_device.hpp
+void rk4_step_launches(heom_handle* h, double dt)
+{
+    // stages alternate their tile walk, so that each starts where the previous one
+    // ended, on the part of its input that is still in the last-level cache
+    launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt, false);
+    launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt, true);
+    launch_stage(h, 3, h->yb.p, h->ya.p, dt, false);
+    launch_stage(h, 4, h->ya.p, h->y.p, dt / 6.0, true);
+}
+
+void drop_graph(heom_handle* h)
+{
+    if (h->step_graph)
+        HIPC(hipGraphExecDestroy(h->step_graph));
+    h->step_graph = nullptr;
+}
+
+// The step graph holds the stage launches with their arguments (pointers to the
+// handle's buffers, the stage coefficients of dt, the kernel variant for real or
+// complex H); it is rebuilt when dt or the variant changes.  It contains kernel
+// launches only.
 void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 {
-    for (int64_t s = 0; s < nsteps; ++s) {
-        // stages alternate their tile walk, so that each starts where the previous one
-        // ended, on the part of its input that is still in the last-level cache
-        launch_stage(h, 1, h->y.p, h->ya.p, 0.5 * dt, false);
-        launch_stage(h, 2, h->ya.p, h->yb.p, 0.5 * dt, true);
-        launch_stage(h, 3, h->yb.p, h->ya.p, dt, false);
-        launch_stage(h, 4, h->ya.p, h->y.p, dt / 6.0, true);
+    if (!h->use_graphs || nsteps < 2) {
+        for (int64_t s = 0; s < nsteps; ++s)
+            rk4_step_launches(h, dt);
+        return;
     }
+    if (h->step_graph && (h->graph_dt != dt || h->graph_real_h != h->real_h))
+        drop_graph(h);
+    if (!h->step_graph) {
+        hipGraph_t g = nullptr;
+        HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        const int64_t launches = h->timing.rhs_launches;
+        try {
+            rk4_step_launches(h, dt);
+        } catch (...) {
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+        }
+        h->timing.rhs_launches = launches;
+        HIPC(hipStreamEndCapture(h->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&h->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPC(e);
+        h->graph_dt = dt;
+        h->graph_real_h = h->real_h;
+    }
+    for (int64_t s = 0; s < nsteps; ++s)
+        HIPC(hipGraphLaunch(h->step_graph, h->stream));
+    h->timing.rhs_launches += 4 * nsteps;
 }
 
 void refold_tables(heom_handle* h)
@@ -551,6 +627,7 @@ void set_hamiltonians(heom_handle* h, const double* H)
     }
     HIPC(hipStreamSynchronize(h->stream));
     h->real_h = real;
+    drop_graph(h);
 }
 
 int count_bath(const heom_handle* h, int s, int b)
@@ -1242,6 +1319,69 @@ int heom_propagate(heom_handle* h, double dt, int64_t nsteps, int64_t observe_st
         HIPC(hipMemcpyAsync(tops, rec.p, rec.bytes(), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         resolve_timing(h);
+    });
+}
+
+int heom_propagate_observe(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps, const double* ops,
+                           double* pops, double* traces)
+{
+    return guard([&] {
+        need(h);
+        if (!pops)
+            raise(HEOM_E_INVALID, "pops is NULL");
+        if (ops && !traces)
+            raise(HEOM_E_INVALID, "traces is NULL while ops is given");
+        if (nsteps < 0 || observe_steps < 1)
+            raise(HEOM_E_INVALID, "nsteps must be >= 0 and observe_steps >= 1");
+        if (!(dt > 0.0))
+            raise(HEOM_E_INVALID, "step size must be > 0");
+        if (h->nel > 64)
+            raise(HEOM_E_UNSUPPORTED, "observations need N*N <= 64");
+        const int64_t rows = nsteps / observe_steps + 1;
+        DevBuf<double> prec;
+        DevBuf<double2> trec, opsd;
+        prec.alloc((size_t)(rows * h->count * h->N));
+        if (ops) {
+            trec.alloc((size_t)(rows * h->count));
+            opsd.alloc((size_t)h->count * h->nel);
+            HIPC(hipMemcpyAsync(opsd.p, ops, opsd.bytes(), hipMemcpyHostToDevice, h->stream));
+        }
+        auto observe = [&](int64_t row) {
+            gather_tops(h, h->small.p);
+            hipLaunchKernelGGL(observe_tops, dim3(h->count), dim3(64), 0, h->stream, h->small.p,
+                               ops ? opsd.p : nullptr, prec.p + row * h->count * h->N,
+                               ops ? trec.p + row * h->count : nullptr, h->N);
+            HIPC(hipGetLastError());
+        };
+        resolve_timing(h);
+        HIPC(hipEventRecord(h->ev_start, h->stream));
+        observe(0);
+        int64_t row = 1, done = 0;
+        while (done < nsteps) {
+            const int64_t n = std::min(observe_steps, nsteps - done);
+            rk4_steps(h, dt, n);
+            done += n;
+            if (n == observe_steps)
+                observe(row++);
+        }
+        HIPC(hipEventRecord(h->ev_stop, h->stream));
+        h->timing_pending = true;
+        h->pending_steps = nsteps;
+        HIPC(hipMemcpyAsync(pops, prec.p, prec.bytes(), hipMemcpyDeviceToHost, h->stream));
+        if (ops)
+            HIPC(hipMemcpyAsync(traces, trec.p, trec.bytes(), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        resolve_timing(h);
+    });
+}
+
+int heom_use_graphs(heom_handle* h, int enable)
+{
+    return guard([&] {
+        need(h);
+        h->use_graphs = enable != 0;
+        if (!h->use_graphs)
+            drop_graph(h);
     });
 }
 

@@ -37,7 +37,7 @@ EXPORTED = [
     "heom_clear_state",
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
-    "heom_rk4_step_bytes", "heom_stage_kernel",
+    "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -108,6 +108,8 @@ def load():
             "heom_device_bytes": [vp, ctypes.POINTER(i64)],
             "heom_rk4_step_bytes": [vp, ctypes.POINTER(i64)],
             "heom_stage_kernel": [vp, ctypes.POINTER(ctypes.c_int)],
+            "heom_propagate_observe": [vp, dbl, i64, i64, vp, vp, vp],
+            "heom_use_graphs": [vp, ctypes.c_int],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -316,6 +318,22 @@ class Handle:
         t = out.reshape(rows, self.count, self.states, self.states, 2)
         t = t[..., 0] + 1j * t[..., 1]
         return t[:, 0] if self.count == 1 else t
+
+    def propagate_observe(self, h: float, nsteps: int, observe_steps: int, ops=None):
+        """Populations (rows, count, N) and, with ops (an N x N operator, or one per
+        instance), traces Tr(ops sigma_0) (rows, count), reduced on the device."""
+        rows = nsteps // observe_steps + 1
+        pops = np.empty(rows * self.count * self.states)
+        traces = np.empty(rows * self.count * 2)
+        o = self._per_instance(ops) if ops is not None else None
+        _check(load().heom_propagate_observe(self._h, float(h), int(nsteps), int(observe_steps),
+                                             o.ctypes.data if o is not None else None, pops.ctypes.data,
+                                             traces.ctypes.data if o is not None else None))
+        tr = traces.reshape(rows, self.count, 2)
+        return pops.reshape(rows, self.count, self.states), (tr[..., 0] + 1j * tr[..., 1] if o is not None else None)
+
+    def use_graphs(self, enable: bool = True) -> None:
+        _check(load().heom_use_graphs(self._h, int(bool(enable))))
 
     def mmult_left(self, mx) -> None:
         m = self._per_instance(mx)

@@ -153,9 +153,22 @@ int heom_step_rk4(heom_handle* handle, double h, int64_t nsteps);
  * 92-106); this does the same without a host round trip per observation. */
 int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                    double* tops /* [nsteps/observe_steps + 1][count][N*N*2] */);
+/* nsteps RK4 steps, observing every instance's top ADO on the device every
+ * observe_steps steps (first record = before stepping): the populations
+ * Re sigma_0[i][i] of dm-heom/include/heom/matrix_diagonal_observer.hpp:28-34 into
+ * pops [rows][count][N] and, when ops [count][N*N*2] is given, the traces
+ * Tr(ops_k sigma_0) of matrix_trace_observer.hpp:139-158 into traces [rows][count][2]
+ * (rows = nsteps / observe_steps + 1).  The reductions run on the device in FP64 (with
+ * FP32 storage, on the FP64 top), one wavefront per instance; N*N <= 64. */
+int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
+                           const double* ops, double* pops, double* traces);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
  * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */
 int heom_mmult_left(heom_handle* handle, const double* mx /* [count][N*N*2] */);
+/* Step through a captured HIP graph of one RK4 step (its stage launches) instead of
+ * launching the stage kernels one by one (no counterpart in the reference; it cuts
+ * the host launch cost per step for small hierarchies and batches).  Off by default. */
+int heom_use_graphs(heom_handle* handle, int enable);
 /* wait for all work queued on the handle */
 int heom_sync(heom_handle* handle);
 int heom_timing(heom_handle* handle, heom_timing_info* info);

@@ -115,3 +115,29 @@ def test_batch_rejects_different_baths(oracle_mod):
     c = instance(6, 3, 3)
     with pytest.raises(heom_amd.HeomError, match="differ"):
         heom_amd.Handle.from_instances([a, c])
+
+
+def test_graph_replays_match_eager(oracle_mod):
+    """One RK4 step captured as a HIP graph, replayed many times with eager work
+    (state uploads, readbacks) in between, gives the eager results bit for bit."""
+    insts = batch_instances()
+    N = insts[0].states
+    top = np.zeros((len(insts), N, N), np.complex128)
+    top[:, 1, 1] = 1.0
+    out = {}
+    for graphs in (False, True):
+        with heom_amd.Handle.from_instances(insts) as h:
+            h.use_graphs(graphs)
+            h.clear_state()
+            h.set_top(top)
+            recs = [h.propagate(1e-15, 30, 10)]
+            s = h.get_state()
+            h.set_state(s)
+            recs.append(h.propagate(1e-15, 30, 10))
+            h.step_rk4(1e-15, 7)
+            recs.append(h.get_top())
+            h.step_rk4(2e-15, 5)  # a new step size rebuilds the graph
+            recs.append(h.get_top())
+            out[graphs] = recs
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(a, b)
