@@ -33,6 +33,7 @@ namespace {
 // cache policy of the table loads: non-temporal (each table row is read once a stage)
 constexpr int kTableCachePolicy = 2;
 
+
 template <typename R, int N>
 struct Shape {
     static constexpr int CW = kStageColumns;
@@ -126,17 +127,6 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
         int2 nbv[NT];
         C xp[NT], xm[NT];
     } L[CW];
-#ifdef HEOM_LDS_NEIGHBOURS
-    // a neighbour inside this tile is read from the LDS copy of the tile; the others
-    // from the element plane (in-tile lanes read the zero slot there, one cached line)
-    const uint32_t zero = (uint32_t)a.zero_slot;
-    auto neighbour = [&](const C* plane, const C* tile_plane, uint32_t p) {
-        const bool in = (p >> 6) == (uint32_t)tile;
-        const C g = plane[in ? zero : p];
-        const C l = tile_plane[p & 63];
-        return in ? l : g;
-    };
-#endif
     // Columns past the row end (N not a multiple of 4) compute a copy of the last column
     // and are not stored: every wavefront runs the same straight-line code, which keeps
     // the per-column arrays in registers.
@@ -150,13 +140,8 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
             L[c].nbv[k] = tab[te[k].mode * kTile + lane];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-#ifdef HEOM_LDS_NEIGHBOURS
-            L[c].xp[k] = neighbour(plane, sig + slot(i, col(c)), (uint32_t)L[c].nbv[k].x >> 6);
-            L[c].xm[k] = neighbour(plane, sig + slot(i, col(c)), (uint32_t)L[c].nbv[k].y);
-#else
             L[c].xp[k] = plane[(uint32_t)L[c].nbv[k].x >> 6];
             L[c].xm[k] = plane[(uint32_t)L[c].nbv[k].y];
-#endif
         }
     };
 
