@@ -1,0 +1,620 @@
+// adjoint.hip -- gradients of observables of the RK4 trajectory by the discrete adjoint
+// (heom_adjoint_forward / heom_adjoint_backward / heom_rhs_adjoint of include/heom_hip.h).
+// The reference has no differentiation; the objective is
+//   J = sum_r Re(w_r X_r),  X_r = Tr(O_k sigma_0(t_r)),  t_r = r * observe_steps * dt,
+// summed over the instances k of the handle.
+//
+// One RK4 step y' = y + dt/6 (k1 + 2 k2 + 2 k3 + k4), k_s = L Y_s, is linear in y, and
+// with lambda = dJ/dy' the adjoint stages
+//   A1 = lambda, A2 = lambda + dt/2 L^dag A1, A3 = lambda + dt/2 L^dag A2, A4 = lambda + dt L^dag A3
+// give dJ/dy = lambda + dt/6 L^dag (A1 + 2 A2 + 2 A3 + A4) -- the same four-stage schedule
+// with L^dag in place of L (stage_kernel_adj.hip) -- and the step's contribution to the
+// gradient with respect to any coefficient kappa of L,
+//   dt/6 <A4, dL/dkappa Y1> + dt/3 <A3, dL/dkappa Y2> + dt/3 <A2, dL/dkappa Y3> + dt/6 <A1, dL/dkappa Y4>,
+// with <a, b> = Re sum conj(a) b (the RHS is complex-linear, so its real transpose is
+// L^dag).  The forward sweep keeps the state every checkpoint_steps steps; the backward
+// sweep recomputes each segment's states from its checkpoint, and each step's forward
+// stage vectors from its state, next to the adjoint stages.
+//
+// The contraction kernel accumulates, per tile of 64 ADOs and in FP64, the sums the
+// gradients of all coefficient arrays of heom_desc follow from:
+//   T_ij  = sum_k Y_ki conj(mu_kj) - conj(mu_ik) Y_jk        (Hamiltonian)
+//   D_ij  = conj(mu_ij) Y_ij                                 (the "same" constants)
+//   P_e,t = i sqrt(n_m + 1) conj(mu_e) Y^{n+e_m}_e           (plus term t of element e)
+//   Q_e,t = sqrt(n_m) conj(mu_e) Y^{n-e_m}_e                 (minus term t of element e)
+// and per ADO sum_e conj(mu_e) Y_e (pf_same_sum); the host maps them onto the arrays
+// through the term constants of build_coefficients (heom_hip.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "handle.hpp"
+#include "heom_device.hpp"
+#include "heom_hip.h"
+
+using namespace heom_hip;
+using namespace heom_rt;
+
+namespace {
+
+struct ContractArgs {
+    const double2* __restrict__ mu;  // adjoint stage vector (planes)
+    const double2* __restrict__ y;   // forward stage vector (planes)
+    const int2* __restrict__ nb;     // neighbour table
+    double2* __restrict__ part;      // [ntiles][nel * (2 + 2 NT)] per-tile sums
+    double2* __restrict__ gs;        // [Mp] per-ADO sums
+    int64_t mp;
+    double w;                        // weight of this stage pair (dt/6 or dt/3)
+};
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_xor(x, off);
+    return x;
+}
+
+// conj(a) * b
+__device__ __forceinline__ double2 cdot(double2 a, double2 b)
+{
+    return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
+}
+
+// Work split as the stage kernel: workgroup = tile of 64 ADOs (lane = ADO), wavefront
+// = 4 columns j of one row i; the tile of both vectors is staged in LDS for T.
+template <int N, int NT>
+__global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(ContractArgs a,
+                                                                        const Term* __restrict__ terms)
+{
+    constexpr int CW = kStageColumns;
+    constexpr int NJW = (N + CW - 1) / CW;
+    constexpr int WAVES = N * NJW;
+    constexpr int SLOTS = 2 + 2 * NT;
+    __shared__ double2 sy[N * N * kTile];
+    __shared__ double2 sm[N * N * kTile];
+    __shared__ double2 red[WAVES * kTile];
+
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = wv / NJW;
+    const int jb = (wv % NJW) * CW;
+    const int tile = blockIdx.x;
+    const int64_t mp = a.mp;
+    const int64_t q = (int64_t)tile * kTile + lane;
+
+    double2 ym[CW], mm[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const int j = jb + c;
+        if (j < N) {
+            ym[c] = a.y[(int64_t)(i * N + j) * mp + q];
+            mm[c] = a.mu[(int64_t)(i * N + j) * mp + q];
+            sy[(i * N + j) * kTile + lane] = ym[c];
+            sm[(i * N + j) * kTile + lane] = mm[c];
+        }
+    }
+    __syncthreads();
+
+    double2 dsum = make_double2(0.0, 0.0);
+    double2* out = a.part + (int64_t)tile * (N * N * SLOTS);
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+        const int j = jb + c;
+        if (j >= N)
+            break;
+        const int e = i * N + j;
+        const double2* plane = a.y + (int64_t)e * mp;
+        double2* o = out + e * SLOTS;
+        double2 tt = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const double2 a1 = cdot(sm[(k * N + j) * kTile + lane], sy[(k * N + i) * kTile + lane]);
+            const double2 a2 = cdot(sm[(i * N + k) * kTile + lane], sy[(j * N + k) * kTile + lane]);
+            tt.x += a1.x - a2.x;
+            tt.y += a1.y - a2.y;
+        }
+        const double2 d = cdot(mm[c], ym[c]);
+        dsum.x += d.x;
+        dsum.y += d.y;
+        const double r0 = wave_sum(tt.x), r1 = wave_sum(tt.y), r2 = wave_sum(d.x), r3 = wave_sum(d.y);
+        if (lane == 0) {
+            o[0] = make_double2(fma(a.w, r0, o[0].x), fma(a.w, r1, o[0].y));
+            o[1] = make_double2(fma(a.w, r2, o[1].x), fma(a.w, r3, o[1].y));
+        }
+        // one term at a time: gathers, products, reductions
+        for (int k = 0; k < NT; ++k) {
+            const int2 t = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
+            const int v = t.x & 63;
+            const double sp = sqrt((double)(v + 1)), sn = sqrt((double)v);
+            const double2 p = cdot(mm[c], plane[(uint32_t)t.x >> 6]);  // times i sqrt(n+1)
+            const double2 m = cdot(mm[c], plane[(uint32_t)t.y]);
+            const double p0 = wave_sum(-sp * p.y), p1 = wave_sum(sp * p.x);
+            const double m0 = wave_sum(sn * m.x), m1 = wave_sum(sn * m.y);
+            if (lane == 0) {
+                o[2 + k] = make_double2(fma(a.w, p0, o[2 + k].x), fma(a.w, p1, o[2 + k].y));
+                o[2 + NT + k] = make_double2(fma(a.w, m0, o[2 + NT + k].x), fma(a.w, m1, o[2 + NT + k].y));
+            }
+        }
+    }
+    red[wv * kTile + lane] = dsum;
+    __syncthreads();
+    if (wv == 0) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int k = 0; k < WAVES; ++k) {
+            t.x += red[k * kTile + lane].x;
+            t.y += red[k * kTile + lane].y;
+        }
+        const double2 g = a.gs[q];
+        a.gs[q] = make_double2(fma(a.w, t.x, g.x), fma(a.w, t.y, g.y));
+    }
+}
+
+// out[k][s] = sum of part[t][s] over the tiles t of instance k (tile_inst; null = one)
+__global__ void sum_tiles(const double2* __restrict__ part, const int32_t* __restrict__ tile_inst,
+                          double2* __restrict__ out, int ntiles, int slots, int count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * slots)
+        return;
+    const int k = t / slots, s = t % slots;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int tile = 0; tile < ntiles; ++tile)
+        if (!tile_inst ? k == 0 : tile_inst[tile] == k) {
+            acc.x += part[(int64_t)tile * slots + s].x;
+            acc.y += part[(int64_t)tile * slots + s].y;
+        }
+    out[t] = acc;
+}
+
+__global__ void conj_copy(const double2* __restrict__ in, double2* __restrict__ out, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+        out[t] = make_double2(in[t].x, -in[t].y);
+}
+
+// -H^dag / hbar per instance from H / hbar
+__global__ void adjoint_hamiltonian(const double2* __restrict__ hs, double2* __restrict__ out, int count, int N)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nel = N * N;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel, i = e / N, j = e % N;
+    const double2 v = hs[(int64_t)k * nel + j * N + i];
+    out[t] = make_double2(-v.x, v.y);
+}
+
+// lambda_top(k) += src[k] for every instance: thread = (k, e)
+__global__ void add_tops(double2* __restrict__ lam, const double2* __restrict__ src, const int64_t* __restrict__ qs,
+                         int count, int64_t mp, int nel)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count * nel)
+        return;
+    const int k = t / nel, e = t % nel;
+    double2* p = lam + (int64_t)e * mp + qs[k];
+    *p = make_double2(p->x + src[t].x, p->y + src[t].y);
+}
+
+template <int N, int NT>
+void launch_contract_t(const ContractArgs& a, const Term* terms, int ntiles, hipStream_t st)
+{
+    hipLaunchKernelGGL((adjoint_contract<N, NT>), dim3(ntiles), dim3(64 * stage_waves(N)), 0, st, a, terms);
+}
+
+template <int N>
+void launch_contract_n(int nt, const ContractArgs& a, const Term* terms, int ntiles, hipStream_t st)
+{
+    switch (nt) {
+    case 2: launch_contract_t<N, 2>(a, terms, ntiles, st); break;
+    case 4: launch_contract_t<N, 4>(a, terms, ntiles, st); break;
+    default: launch_contract_t<N, 8>(a, terms, ntiles, st); break;
+    }
+}
+
+void launch_contract(heom_handle* h, const void* mu, const void* y, double w)
+{
+    AdjointState& A = *h->adj;
+    ContractArgs a{};
+    a.mu = reinterpret_cast<const double2*>(mu);
+    a.y = reinterpret_cast<const double2*>(y);
+    a.nb = h->nb.p;
+    a.part = A.part.p;
+    a.gs = A.gs.p;
+    a.mp = h->Mp;
+    a.w = w;
+    const int nt = h->term_slots, nb = h->ntiles;
+    switch (h->N) {
+    case 1: launch_contract_n<1>(nt, a, h->terms.p, nb, h->stream); break;
+    case 2: launch_contract_n<2>(nt, a, h->terms.p, nb, h->stream); break;
+    case 3: launch_contract_n<3>(nt, a, h->terms.p, nb, h->stream); break;
+    case 4: launch_contract_n<4>(nt, a, h->terms.p, nb, h->stream); break;
+    case 5: launch_contract_n<5>(nt, a, h->terms.p, nb, h->stream); break;
+    case 6: launch_contract_n<6>(nt, a, h->terms.p, nb, h->stream); break;
+    case 7: launch_contract_n<7>(nt, a, h->terms.p, nb, h->stream); break;
+    default: launch_contract_n<8>(nt, a, h->terms.p, nb, h->stream); break;
+    }
+    HIPC(hipGetLastError());
+}
+
+AdjointState& adjoint_state(heom_handle* h)
+{
+    if (!h->adj)
+        h->adj.reset(new AdjointState());
+    return *h->adj;
+}
+
+void check_adjoint_supported(heom_handle* h)
+{
+    if (h->fp32())
+        raise(HEOM_E_UNSUPPORTED, "the adjoint needs FP64 storage");
+    if (h->nel > 64)
+        raise(HEOM_E_UNSUPPORTED, "the adjoint needs N*N <= 64");
+    // the transpose of a filtered hierarchy masks destinations instead of sources;
+    // the adjoint kernels implement the unfiltered operator
+    std::vector<uint8_t> mask(h->M);
+    HIPC(hipMemcpyAsync(mask.data(), h->mask.p, h->M, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int64_t u = 0; u < h->M; ++u)
+        if (mask[u] == 0)
+            raise(HEOM_E_UNSUPPORTED, "the adjoint needs an unfiltered hierarchy (no masked ADOs)");
+}
+
+// element constants of L^dag: -H^dag / hbar, conj(gam), conj(pf_same_sum), and per term
+// -plus_c and conj(minus_c) (the kernel exchanges the edges they multiply)
+void prepare_operator(heom_handle* h)
+{
+    AdjointState& A = adjoint_state(h);
+    if (A.ops_ready)
+        return;
+    const int n = h->count * h->nel;
+    A.hs.alloc(n);
+    hipLaunchKernelGGL(adjoint_hamiltonian, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->hs.p, A.hs.p,
+                       h->count, h->N);
+    A.gam.alloc(h->nel);
+    hipLaunchKernelGGL(conj_copy, dim3(blocks_for(h->nel, 256)), dim3(256), 0, h->stream, h->gam.p, A.gam.p,
+                       (int64_t)h->nel);
+    A.pfss.alloc(h->Mp);
+    hipLaunchKernelGGL(conj_copy, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, h->pfss.p, A.pfss.p,
+                       h->Mp);
+    HIPC(hipGetLastError());
+    std::vector<Term> t((size_t)h->nel * h->term_slots);
+    HIPC(hipMemcpyAsync(t.data(), h->terms.p, t.size() * sizeof(Term), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (Term& x : t) {
+        x.plus_c = -x.plus_c;
+        x.minus_c.y = -x.minus_c.y;
+    }
+    A.terms.alloc(t.size());
+    HIPC(hipMemcpyAsync(A.terms.p, t.data(), t.size() * sizeof(Term), hipMemcpyHostToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    A.real_h = h->real_h;
+    A.ops_ready = true;
+}
+
+size_t plane_bytes(const heom_handle* h) { return h->y.bytes(); }
+
+unsigned char* slot(DevBuf<unsigned char>& b, size_t bytes, int64_t k) { return b.p + (size_t)k * bytes; }
+
+// dst planes = one RK4 step of src (FP64, explicit buffers); f2 / f3 / f4 receive Y2, Y3, Y4
+void rk4_step_into(heom_handle* h, const unsigned char* src, unsigned char* dst, double dt)
+{
+    AdjointState& A = *h->adj;
+    launch_stage_buffers(h, 1, src, src, h->pbuf.p, A.f2.p, 0.5 * dt, false, false);
+    launch_stage_buffers(h, 2, A.f2.p, src, h->pbuf.p, A.f3.p, 0.5 * dt, false, false);
+    launch_stage_buffers(h, 3, A.f3.p, src, h->pbuf.p, A.f4.p, dt, false, false);
+    if (dst)
+        launch_stage_buffers(h, 4, A.f4.p, src, h->pbuf.p, dst, dt / 6.0, false, false);
+}
+
+// lambda_top += the adjoint source of observation row `row` (rows with all weights zero
+// are skipped on the host)
+void add_source(heom_handle* h, const std::vector<char>& nonzero, int64_t row)
+{
+    if (!nonzero[row])
+        return;
+    AdjointState& A = *h->adj;
+    const int n = h->count * h->nel;
+    hipLaunchKernelGGL(add_tops, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, A.lam.as<double2>(),
+                       A.src.p + (size_t)row * n, h->q_top.p, h->count, h->Mp, h->nel);
+    HIPC(hipGetLastError());
+}
+
+}  // namespace
+
+namespace heom_rt {
+
+void adjoint_invalidate(heom_handle* h)
+{
+    if (h->adj) {
+        h->adj->ops_ready = false;
+        h->adj->recorded = false;
+    }
+}
+
+}  // namespace heom_rt
+
+extern "C" {
+
+int heom_rhs_adjoint(heom_handle* h, double* out)
+{
+    return guard([&] {
+        need(h);
+        if (!out)
+            raise(HEOM_E_INVALID, "out is NULL");
+        check_adjoint_supported(h);
+        prepare_operator(h);
+        launch_stage_buffers(h, 0, h->y.p, h->y.p, h->pbuf.p, h->ya.p, 0.0, false, true);
+        download_state(h, h->ya.p, nullptr, out);
+    });
+}
+
+int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps, int64_t checkpoint_steps,
+                         const double* ops, double* observed)
+{
+    return guard([&] {
+        need(h);
+        if (!ops || !observed)
+            raise(HEOM_E_INVALID, "ops and observed must not be NULL");
+        if (nsteps < 0 || observe_steps < 1 || checkpoint_steps < 1)
+            raise(HEOM_E_INVALID, "nsteps must be >= 0, observe_steps and checkpoint_steps >= 1");
+        if (!(dt > 0.0))
+            raise(HEOM_E_INVALID, "step size must be > 0");
+        check_adjoint_supported(h);
+        AdjointState& A = adjoint_state(h);
+        A.recorded = false;
+        const int64_t rows = nsteps / observe_steps + 1;
+        const int64_t last = (rows - 1) * observe_steps;
+        const int64_t ncp = last / checkpoint_steps + 1;
+        const size_t pb = plane_bytes(h);
+        A.cps.alloc(pb * (size_t)ncp);
+        const int n = h->count * h->nel;
+        A.op.assign(ops, ops + 2 * (size_t)n);
+        DevBuf<double2> opsd, trec;
+        DevBuf<double> prec;
+        opsd.alloc(n);
+        trec.alloc((size_t)(rows * h->count));
+        prec.alloc((size_t)(rows * h->count * h->N));
+        HIPC(hipMemcpyAsync(opsd.p, ops, n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        resolve_timing(h);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        try {
+            HIPC(hipEventRecord(e0, h->stream));
+            observe_launch(h, opsd.p, prec.p, trec.p);
+            HIPC(hipMemcpyAsync(A.cps.p, h->y.p, pb, hipMemcpyDeviceToDevice, h->stream));
+            int64_t done = 0;
+            while (done < last) {
+                // advance to the next observation or checkpoint, whichever comes first
+                const int64_t to_obs = observe_steps - done % observe_steps;
+                const int64_t to_cp = checkpoint_steps - done % checkpoint_steps;
+                const int64_t n_adv = std::min(std::min(to_obs, to_cp), last - done);
+                rk4_steps(h, dt, n_adv);
+                done += n_adv;
+                if (done % observe_steps == 0)
+                    observe_launch(h, opsd.p, prec.p + (done / observe_steps) * h->count * h->N,
+                                   trec.p + (done / observe_steps) * h->count);
+                if (done % checkpoint_steps == 0)
+                    HIPC(hipMemcpyAsync(slot(A.cps, pb, done / checkpoint_steps), h->y.p, pb,
+                                        hipMemcpyDeviceToDevice, h->stream));
+            }
+            if (nsteps > last)
+                rk4_steps(h, dt, nsteps - last);
+            HIPC(hipEventRecord(e1, h->stream));
+            HIPC(hipMemcpyAsync(observed, trec.p, trec.bytes(), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, e0, e1));
+            h->timing.last_call_seconds = ms * 1e-3;
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        A.dt = dt;
+        A.nsteps = nsteps;
+        A.observe_steps = observe_steps;
+        A.checkpoint_steps = checkpoint_steps;
+        A.rows = rows;
+        A.last = last;
+        A.ncp = ncp;
+        A.recorded = true;
+    });
+}
+
+int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients* g)
+{
+    return guard([&] {
+        need(h);
+        if (!weights || !g)
+            raise(HEOM_E_INVALID, "weights and grads must not be NULL");
+        if (!h->adj || !h->adj->recorded)
+            raise(HEOM_E_INVALID, "no forward sweep on record (heom_adjoint_forward), or the operator changed since");
+        check_adjoint_supported(h);
+        prepare_operator(h);
+        AdjointState& A = *h->adj;
+        const int N = h->N, nel = h->nel, count = h->count, NT = h->term_slots;
+        const int slots = nel * (2 + 2 * NT);
+        const size_t pb = plane_bytes(h);
+        const int64_t ck = A.checkpoint_steps;
+        const double dt = A.dt;
+
+        // adjoint sources conj(w_rk) O_k^dag, [rows][count][nel], resident on the device
+        std::vector<double2> src((size_t)A.rows * count * nel);
+        std::vector<char> nonzero(A.rows, 0);
+        for (int64_t r = 0; r < A.rows; ++r)
+            for (int k = 0; k < count; ++k) {
+                const std::complex<double> w(weights[2 * (r * count + k)], weights[2 * (r * count + k) + 1]);
+                if (w != 0.0)
+                    nonzero[r] = 1;
+                for (int i = 0; i < N; ++i)
+                    for (int j = 0; j < N; ++j) {
+                        const size_t o = ((size_t)k * nel + j * N + i) * 2;
+                        const std::complex<double> v = std::conj(w * std::complex<double>(A.op[o], A.op[o + 1]));
+                        src[((size_t)r * count + k) * nel + i * N + j] = make_double2(v.real(), v.imag());
+                    }
+            }
+
+        A.seg.alloc(pb * (size_t)std::min<int64_t>(ck, std::max<int64_t>(A.last, 1)));
+        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b})
+            b->alloc(pb);
+        A.part.alloc((size_t)h->ntiles * slots);
+        A.gs.alloc(h->Mp);
+        A.src.alloc(src.size());
+        HIPC(hipMemcpyAsync(A.src.p, src.data(), src.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+        // padding slots of the state planes must read as zero
+        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.seg})
+            HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+        HIPC(hipMemsetAsync(A.part.p, 0, A.part.bytes(), h->stream));
+        HIPC(hipMemsetAsync(A.gs.p, 0, A.gs.bytes(), h->stream));
+
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        try {
+            HIPC(hipEventRecord(e0, h->stream));
+            add_source(h, nonzero, A.rows - 1);
+            for (int64_t c = A.last > 0 ? (A.last - 1) / ck : -1; c >= 0; --c) {
+                const int64_t start = c * ck, end = std::min(start + ck, A.last);
+                HIPC(hipMemcpyAsync(A.seg.p, slot(A.cps, pb, c), pb, hipMemcpyDeviceToDevice, h->stream));
+                for (int64_t n = start + 1; n < end; ++n)
+                    rk4_step_into(h, slot(A.seg, pb, n - start - 1), slot(A.seg, pb, n - start), dt);
+                for (int64_t n = end - 1; n >= start; --n) {
+                    unsigned char* yn = slot(A.seg, pb, n - start);
+                    rk4_step_into(h, yn, nullptr, dt);  // Y2, Y3, Y4 of step n
+                    launch_stage_buffers(h, 1, A.lam.p, A.lam.p, h->pbuf.p, A.a.p, 0.5 * dt, false, true);
+                    launch_contract(h, A.lam.p, A.f4.p, dt / 6.0);
+                    launch_stage_buffers(h, 2, A.a.p, A.lam.p, h->pbuf.p, A.b.p, 0.5 * dt, false, true);
+                    launch_contract(h, A.a.p, A.f3.p, dt / 3.0);
+                    launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.a.p, dt, false, true);
+                    launch_contract(h, A.b.p, A.f2.p, dt / 3.0);
+                    launch_contract(h, A.a.p, yn, dt / 6.0);
+                    launch_stage_buffers(h, 4, A.a.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
+                    if (n % A.observe_steps == 0)
+                        add_source(h, nonzero, n / A.observe_steps);
+                }
+            }
+            HIPC(hipEventRecord(e1, h->stream));
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+
+        // reduce the tiles per instance
+        DevBuf<double2> red;
+        red.alloc((size_t)count * slots);
+        hipLaunchKernelGGL(sum_tiles, dim3(blocks_for((int64_t)count * slots, 256)), dim3(256), 0, h->stream,
+                           A.part.p, count > 1 ? h->tile_inst.p : nullptr, red.p, h->ntiles, slots, count);
+        HIPC(hipGetLastError());
+        std::vector<double2> sums((size_t)count * slots), gs(h->Mp);
+        std::vector<int32_t> q2u(h->Mp);
+        std::vector<Term> terms((size_t)nel * NT);
+        HIPC(hipMemcpyAsync(sums.data(), red.p, red.bytes(), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(gs.data(), A.gs.p, A.gs.bytes(), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(q2u.data(), h->q2u.p, q2u.size() * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(terms.data(), h->terms.p, terms.size() * sizeof(Term), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        h->timing.last_call_seconds = ms * 1e-3;
+
+        using cd = std::complex<double>;
+        auto at = [&](int k, int e, int s) {
+            const double2 v = sums[(size_t)k * slots + e * (2 + 2 * NT) + s];
+            return cd(v.x, v.y);
+        };
+        // Hamiltonian: delta J = Re sum dH_ij (i/hbar) T_ij, so dJ/dRe + i dJ/dIm = conj(i T / hbar)
+        if (g->hamiltonian)
+            for (int k = 0; k < count; ++k)
+                for (int e = 0; e < nel; ++e) {
+                    const cd v = std::conj(cd(0.0, 1.0) * at(k, e, 0) / kHbar);
+                    g->hamiltonian[2 * ((size_t)k * nel + e)] = v.real();
+                    g->hamiltonian[2 * ((size_t)k * nel + e) + 1] = v.imag();
+                }
+        // pf_same_sum: S_n multiplies sigma_n, delta J = Re dS_n sum conj(mu) Y
+        if (g->pf_same_sum) {
+            std::fill(g->pf_same_sum, g->pf_same_sum + 2 * h->M, 0.0);
+            for (int64_t q = 0; q < h->Mp; ++q)
+                if (q2u[q] >= 0) {
+                    g->pf_same_sum[2 * (size_t)q2u[q]] = gs[q].x;
+                    g->pf_same_sum[2 * (size_t)q2u[q] + 1] = -gs[q].y;
+                }
+        }
+        // bath arrays, summed over the instances: gam_ij = -sum Delta_b over the coupling
+        // entries of i and j (i != j); plus_c = (ci - cj) sqrt(|c|/hbar) and
+        // minus_c = ((ci - cj) i c / hbar + [k = 0] (ci + cj) Phi_b) sqrt(hbar / |c|)
+        std::vector<cd> g_same(h->B), g_circ(h->B), g_cbk((size_t)h->B * h->K);
+        std::vector<double> d_cr((size_t)h->B * h->K, 0.0), d_ci((size_t)h->B * h->K, 0.0), d_fr(h->B, 0.0),
+            d_fi(h->B, 0.0);
+        for (int e = 0; e < nel; ++e) {
+            const int i = e / N, j = e % N;
+            cd gd = 0.0;
+            for (int k = 0; k < count; ++k)
+                gd += at(k, e, 1);
+            if (i != j)
+                for (int b = 0; b < h->B; ++b) {
+                    const int cnt = count_bath(h, i, b) + count_bath(h, j, b);
+                    g_same[b] -= std::conj(gd) * (double)cnt;
+                }
+            const int nterms = terms[(size_t)e * NT].info & 0xff;
+            for (int t = 0; t < nterms; ++t) {
+                const Term& tm = terms[(size_t)e * NT + t];
+                cd gp = 0.0, gm = 0.0;
+                for (int k = 0; k < count; ++k) {
+                    gp += at(k, e, 2 + t);
+                    gm += at(k, e, 2 + NT + t);
+                }
+                const int m = tm.mode, b = m / h->K, kk = m % h->K;
+                const int ci = count_bath(h, i, b), cj = count_bath(h, j, b);
+                const double cr = h->cbk[2 * m], cim = h->cbk[2 * m + 1];
+                const double a = std::hypot(cr, cim);
+                const double s = std::sqrt(kHbar / a);
+                const double D = ci - cj, Sg = ci + cj;
+                cd X = D * cd(-cim / kHbar, cr / kHbar);
+                if (kk == 0)
+                    X += Sg * cd(h->pf_circ[2 * b], h->pf_circ[2 * b + 1]);
+                const double dp = D / (2.0 * std::sqrt(a * kHbar)) / a;  // times cr or cim
+                const double ds = -s / (2.0 * a * a);                     // times cr or cim
+                const cd dm_cr = D * cd(0.0, 1.0 / kHbar) * s + X * ds * cr;
+                const cd dm_ci = -D / kHbar * s + X * ds * cim;
+                d_cr[m] += gp.real() * dp * cr + (gm * dm_cr).real();
+                d_ci[m] += gp.real() * dp * cim + (gm * dm_ci).real();
+                if (kk == 0) {
+                    d_fr[b] += (gm * (Sg * s)).real();
+                    d_fi[b] += (gm * cd(0.0, Sg * s)).real();
+                }
+            }
+        }
+        if (g->pf_same)
+            for (int b = 0; b < h->B; ++b) {
+                g->pf_same[2 * b] = g_same[b].real();
+                g->pf_same[2 * b + 1] = g_same[b].imag();
+            }
+        if (g->pf_circ)
+            for (int b = 0; b < h->B; ++b) {
+                g->pf_circ[2 * b] = d_fr[b];
+                g->pf_circ[2 * b + 1] = d_fi[b];
+            }
+        if (g->cbk)
+            for (int m = 0; m < h->B * h->K; ++m) {
+                g->cbk[2 * m] = d_cr[m];
+                g->cbk[2 * m + 1] = d_ci[m];
+            }
+        if (g->state)
+            download_state(h, A.lam.p, nullptr, g->state);
+    });
+}
+
+}  // extern "C"

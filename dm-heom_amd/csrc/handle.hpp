@@ -1,0 +1,206 @@
+// handle.hpp -- the handle of include/heom_hip.h and the host-side helpers that the
+// translation units of the C ABI share (heom_hip.hip: handles, state, RK4 driver;
+// adjoint.hip: the adjoint sweep).  Host code only; no kernels.
+#ifndef HEOM_HANDLE_HPP
+#define HEOM_HANDLE_HPP
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "heom_device.hpp"
+#include "heom_hip.h"
+
+namespace heom_rt {
+
+constexpr double kHbar = 1.05457148e-34;  // dm-heom/cl/heom/common.cl:45
+
+// the message heom_last_error returns (defined in heom_hip.hip)
+extern thread_local std::string g_error;
+
+struct heom_error : std::runtime_error {
+    int code;
+    heom_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void raise(int code, const std::string& msg) { throw heom_error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        raise(HEOM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPC(x) ::heom_rt::hip_check((x), #x)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { free(); }
+    void alloc(size_t count)
+    {
+        free();
+        if (count == 0)
+            count = 1;
+        HIPC(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void free()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+    template <typename U>
+    U* as() const
+    {
+        return reinterpret_cast<U*>(p);
+    }
+};
+
+// The adjoint (adjoint.hip): the recorded forward sweep (checkpoints), the element
+// constants of L^dag, and the buffers of the backward sweep.
+struct AdjointState {
+    bool recorded = false;  // a forward sweep is on record for heom_adjoint_backward
+    double dt = 0.0;
+    int64_t nsteps = 0, observe_steps = 1, checkpoint_steps = 1;
+    int64_t rows = 0;       // observations, t_r = r * observe_steps * dt
+    int64_t last = 0;       // step of the last observation
+    int64_t ncp = 0;        // checkpoints, at steps 0, k, 2k, ... <= last
+    std::vector<double> op; // [nel * 2] the observable O
+    DevBuf<unsigned char> cps;  // [ncp] state planes
+
+    bool ops_ready = false;  // the L^dag constants below match the handle's operator
+    bool real_h = false;
+    DevBuf<double2> hs, gam, pfss;  // -H^dag / hbar, conj(gam), conj(pf_same_sum)
+    DevBuf<heom_hip::Term> terms;   // conj(minus_c) and -plus_c per term slot
+
+    DevBuf<unsigned char> seg;                   // [checkpoint_steps] states y_n of a segment
+    DevBuf<unsigned char> f2, f3, f4;            // forward stage vectors Y2, Y3, Y4
+    DevBuf<unsigned char> lam, a, b;             // adjoint state and stage vectors
+    DevBuf<double2> part;                        // [ntiles][slots] per-tile gradient sums
+    DevBuf<double2> gs;                          // [Mp] per-ADO d J / d pf_same_sum
+    DevBuf<double2> src;                         // [rows][count][nel] adjoint sources conj(w_r) O^dag
+};
+
+}  // namespace heom_rt
+
+struct heom_handle {
+    template <typename T>
+    using DevBuf = heom_rt::DevBuf<T>;
+    using Term = heom_hip::Term;
+    template <typename R>
+    using TermT = heom_hip::TermT<R>;
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int N = 0, K = 0, B = 0, BM = 0, W = 0, nel = 0;
+    int prec = HEOM_PRECISION_FP64;
+    int count = 1;                  // instances in the batch
+    std::vector<int64_t> inst_M;    // ADOs per instance
+    std::vector<int64_t> inst_u;    // first global uid of each instance
+    std::vector<int64_t> inst_q;    // first internal index (a multiple of 64)
+    int64_t M = 0, Mp = 0;          // total ADOs (uid space), internal slots
+    int ntiles = 0;
+    int32_t zero_slot = 0;
+    bool real_h = false;
+    int term_slots = 2;      // neighbour terms per element the stage kernel runs with
+    bool pipelined = false;  // stage_pipelined.hip (persistent) instead of stage_kernel.hip
+    int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
+    int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
+
+    std::vector<int32_t> coupling;  // [N * BM]
+    std::vector<double> cbk, pf_same, pf_circ;
+
+    DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
+    DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded
+    DevBuf<int32_t> q2u, tile_inst;
+    DevBuf<int64_t> q_top;                  // [count] internal index of each top ADO
+    DevBuf<uint8_t> mask;
+    DevBuf<double2> pfss, hs, gam, mx, small;  // FP64 constants
+    DevBuf<Term> terms;
+    DevBuf<double> norms;
+    // FP32 storage
+    DevBuf<float2> pfss32, hs32, gam32;
+    DevBuf<TermT<float>> terms32;
+    int S = 0;                    // tier-0/1 ADOs
+    std::vector<int32_t> sp_top;  // [count] tier-0/1 index of each top
+    std::vector<int64_t> sp_uid;  // [S] global uid
+    DevBuf<int64_t> sp_q;
+    DevBuf<int32_t> sp_inst, sp_top_d;
+    DevBuf<int2> sp_nb;
+    DevBuf<double2> y64, sa64, sb64, p64, sp_pfss, sp_tmp;
+
+    // one RK4 step captured as a HIP graph (heom_use_graphs), replayed per step
+    bool use_graphs = false;
+    hipGraphExec_t step_graph = nullptr;
+    double graph_dt = 0.0;
+    bool graph_real_h = false;
+
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timing_pending = false;
+    int64_t pending_steps = 0;
+    heom_timing_info timing{};
+
+    // the recorded forward sweep and buffers of the adjoint (adjoint.hip)
+    std::unique_ptr<heom_rt::AdjointState> adj;
+
+    bool fp32() const { return prec == HEOM_PRECISION_FP32; }
+    size_t cbytes() const { return fp32() ? sizeof(float2) : sizeof(double2); }
+
+    heom_handle();
+    ~heom_handle();
+    void release()
+    {
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        if (step_graph)
+            (void)hipGraphExecDestroy(step_graph);
+        if (ev_start)
+            (void)hipEventDestroy(ev_start);
+        if (ev_stop)
+            (void)hipEventDestroy(ev_stop);
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace heom_rt {
+
+// error boundary of every C ABI function: the status code, with g_error set
+int guard(const std::function<void()>& fn);
+void need(heom_handle* h);
+inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// one stage launch of the FP64 RK4 schedule with explicit buffers: yin the stage input
+// (neighbours are read there), y0 = y_n (stages 2, 3), pbuf = P (written by stage 2, read
+// by stage 4), yout the stage output; adjoint = true applies L^dag instead of L
+void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void* y0, void* pbuf, void* yout,
+                          double c1, bool reverse, bool adjoint);
+// nsteps RK4 steps of the handle's state (through the captured step graph when enabled)
+void rk4_steps(heom_handle* h, double dt, int64_t nsteps);
+// the tops of all instances [count][nel] into dst (device)
+void gather_tops(heom_handle* h, double2* dst);
+// Re sigma_0[i][i] and, with ops, Tr(ops_k sigma_0) of every instance's top into pops
+// [count][N] and traces [count] (device pointers)
+void observe_launch(heom_handle* h, const double2* ops, double* pops, double2* traces);
+// planes in the internal layout -> reference layout (state [M*N*N*2], host)
+void download_state(heom_handle* h, const unsigned char* src, const double2* rows, double* state);
+int count_bath(const heom_handle* h, int s, int b);
+void resolve_timing(heom_handle* h);
+// adjoint.hip: drops the recorded sweep when the operator changes
+void adjoint_invalidate(heom_handle* h);
+
+}  // namespace heom_rt
+
+#endif  // HEOM_HANDLE_HPP

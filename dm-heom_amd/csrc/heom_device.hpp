@@ -260,6 +260,12 @@ void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArg
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
                   const ElementArgsT<float>& el, hipStream_t s);
 
+// the same stage schedule with the adjoint operator L^dag (stage_kernel_adj.hip); el holds
+// the adjoint element constants (-H^dag / hbar, conj(gam), terms with -plus_c and
+// conj(minus_c)) and a.pfss conj(pf_same_sum)
+void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                          const ElementArgsT<double>& el, hipStream_t s);
+
 // ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
 
 // LDS row stride (element slots per matrix row) of the pipelined kernel's tile: the

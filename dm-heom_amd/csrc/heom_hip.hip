@@ -22,30 +22,20 @@
 #include <string>
 #include <vector>
 
+#include "handle.hpp"
 #include "heom_device.hpp"
 #include "heom_hip.h"
 
 using namespace heom_hip;
+using namespace heom_rt;
+
+thread_local std::string heom_rt::g_error;
+
+heom_handle::heom_handle() = default;
+heom_handle::~heom_handle() { release(); }
 
 namespace {
 
-constexpr double kHbar = 1.05457148e-34;  // dm-heom/cl/heom/common.cl:45
-
-thread_local std::string g_error;
-
-struct heom_error : std::runtime_error {
-    int code;
-    heom_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-[[noreturn]] void raise(int code, const std::string& msg) { throw heom_error(code, msg); }
-
-void hip_check(hipError_t e, const char* what)
-{
-    if (e != hipSuccess)
-        raise(HEOM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPC(x) hip_check((x), #x)
 
 // ------------------------------------------------------------------------------------
 // layout, reduction and hierarchy-arithmetic kernels (C = double2 or float2 planes)
@@ -330,113 +320,11 @@ __global__ void tier_fixup(TierArgs a)
     }
 }
 
-// ------------------------------------------------------------------------------------
-// device buffers
-// ------------------------------------------------------------------------------------
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { free(); }
-    void alloc(size_t count)
-    {
-        free();
-        if (count == 0)
-            count = 1;
-        HIPC(hipMalloc(&p, count * sizeof(T)));
-        n = count;
-    }
-    void free()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    size_t bytes() const { return n * sizeof(T); }
-    template <typename U>
-    U* as() const
-    {
-        return reinterpret_cast<U*>(p);
-    }
-};
 
 }  // namespace
 
-struct heom_handle {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int N = 0, K = 0, B = 0, BM = 0, W = 0, nel = 0;
-    int prec = HEOM_PRECISION_FP64;
-    int count = 1;                  // instances in the batch
-    std::vector<int64_t> inst_M;    // ADOs per instance
-    std::vector<int64_t> inst_u;    // first global uid of each instance
-    std::vector<int64_t> inst_q;    // first internal index (a multiple of 64)
-    int64_t M = 0, Mp = 0;          // total ADOs (uid space), internal slots
-    int ntiles = 0;
-    int32_t zero_slot = 0;
-    bool real_h = false;
-    int term_slots = 2;      // neighbour terms per element the stage kernel runs with
-    bool pipelined = false;  // stage_pipelined.hip (persistent) instead of stage_kernel.hip
-    int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
-    int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
 
-    std::vector<int32_t> coupling;  // [N * BM]
-    std::vector<double> cbk, pf_same, pf_circ;
-
-    DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
-    DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded
-    DevBuf<int32_t> q2u, tile_inst;
-    DevBuf<int64_t> q_top;                  // [count] internal index of each top ADO
-    DevBuf<uint8_t> mask;
-    DevBuf<double2> pfss, hs, gam, mx, small;  // FP64 constants
-    DevBuf<Term> terms;
-    DevBuf<double> norms;
-    // FP32 storage
-    DevBuf<float2> pfss32, hs32, gam32;
-    DevBuf<TermT<float>> terms32;
-    int S = 0;                    // tier-0/1 ADOs
-    std::vector<int32_t> sp_top;  // [count] tier-0/1 index of each top
-    std::vector<int64_t> sp_uid;  // [S] global uid
-    DevBuf<int64_t> sp_q;
-    DevBuf<int32_t> sp_inst, sp_top_d;
-    DevBuf<int2> sp_nb;
-    DevBuf<double2> y64, sa64, sb64, p64, sp_pfss, sp_tmp;
-
-    // one RK4 step captured as a HIP graph (heom_use_graphs), replayed per step
-    bool use_graphs = false;
-    hipGraphExec_t step_graph = nullptr;
-    double graph_dt = 0.0;
-    bool graph_real_h = false;
-
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-    bool timing_pending = false;
-    int64_t pending_steps = 0;
-    heom_timing_info timing{};
-
-    bool fp32() const { return prec == HEOM_PRECISION_FP32; }
-    size_t cbytes() const { return fp32() ? sizeof(float2) : sizeof(double2); }
-
-    ~heom_handle()
-    {
-        if (stream)
-            (void)hipStreamSynchronize(stream);
-        if (step_graph)
-            (void)hipGraphExecDestroy(step_graph);
-        if (ev_start)
-            (void)hipEventDestroy(ev_start);
-        if (ev_stop)
-            (void)hipEventDestroy(ev_stop);
-        if (stream)
-            (void)hipStreamDestroy(stream);
-    }
-};
-
-namespace {
+namespace heom_rt {
 
 template <typename T>
 void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
@@ -444,8 +332,6 @@ void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
     d.alloc(h.size());
     HIPC(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
-
-unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 void resolve_timing(heom_handle* h)
 {
@@ -459,6 +345,28 @@ void resolve_timing(heom_handle* h)
     h->timing.rk4_steps += h->pending_steps;
     h->timing_pending = false;
     h->pending_steps = 0;
+}
+
+template <typename R>
+StageArgsT<R> stage_args(heom_handle* h, const void* yin, const void* y0, void* pbuf, void* yout, double c1,
+                         bool reverse, const c2<R>* pfss)
+{
+    StageArgsT<R> a{};
+    a.yin = reinterpret_cast<const c2<R>*>(yin);
+    a.y0 = reinterpret_cast<const c2<R>*>(y0);
+    a.pbuf = reinterpret_cast<c2<R>*>(pbuf);
+    a.yout = reinterpret_cast<c2<R>*>(yout);
+    a.nb = h->nb.p;
+    a.pfss = pfss;
+    a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
+    a.mp = h->Mp;
+    a.ntiles = h->ntiles;
+    a.zero_slot = h->zero_slot;
+    a.table_units = (h->W + 2) / 2;
+    a.xcd_chunk = h->xcd_chunk;
+    a.reverse = reverse ? 1 : 0;
+    a.c1 = (R)c1;
+    return a;
 }
 
 template <typename R>
@@ -485,6 +393,28 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     else
         heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
     HIPC(hipGetLastError());
+}
+
+void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void* y0, void* pbuf, void* yout,
+                          double c1, bool reverse, bool adjoint)
+{
+    if (h->fp32())
+        raise(HEOM_E_UNSUPPORTED, "explicit-buffer stages need FP64 storage");
+    if (adjoint) {
+        AdjointState& A = *h->adj;
+        const StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, A.pfss.p);
+        const ElementArgsT<double> el{A.hs.p, A.gam.p, A.terms.p};
+        heom_hip::launch_stage_adjoint(h->N, h->term_slots, A.real_h, stage, a, el, h->stream);
+    } else {
+        const StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, h->pfss.p);
+        const ElementArgsT<double> el{h->hs.p, h->gam.p, h->terms.p};
+        if (h->pipelined)
+            heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
+        else
+            heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+    }
+    HIPC(hipGetLastError());
+    h->timing.rhs_launches += 1;
 }
 
 // the FP64 pass over the tier-0/1 ADOs after a complex64 stage kernel
@@ -628,6 +558,7 @@ void set_hamiltonians(heom_handle* h, const double* H)
     HIPC(hipStreamSynchronize(h->stream));
     h->real_h = real;
     drop_graph(h);
+    adjoint_invalidate(h);
 }
 
 int count_bath(const heom_handle* h, int s, int b)
@@ -1064,6 +995,13 @@ void gather_tops(heom_handle* h, double2* dst)
     HIPC(hipGetLastError());
 }
 
+void observe_launch(heom_handle* h, const double2* ops, double* pops, double2* traces)
+{
+    gather_tops(h, h->small.p);
+    hipLaunchKernelGGL(observe_tops, dim3(h->count), dim3(64), 0, h->stream, h->small.p, ops, pops, traces, h->N);
+    HIPC(hipGetLastError());
+}
+
 template <typename C>
 void mmult_left_planes(heom_handle* h)
 {
@@ -1083,7 +1021,7 @@ void mmult_left_planes(heom_handle* h)
     HIPC(hipGetLastError());
 }
 
-}  // namespace
+}  // namespace heom_rt
 
 extern "C" {
 
@@ -1156,6 +1094,7 @@ int heom_set_mask(heom_handle* h, const uint8_t* mask)
                 raise(HEOM_E_INVALID, "mask values must be 0, 1 or 2");
         HIPC(hipMemcpyAsync(h->mask.p, mask, h->M, hipMemcpyHostToDevice, h->stream));
         refold_tables(h);
+        adjoint_invalidate(h);
         HIPC(hipStreamSynchronize(h->stream));
     });
 }
@@ -1251,8 +1190,10 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
                                h->stream, h->y.as<double2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p,
                                h->mask.p);
         HIPC(hipGetLastError());
-        if (threshold >= 0.0)
+        if (threshold >= 0.0) {
             refold_tables(h);
+            adjoint_invalidate(h);
+        }
         HIPC(hipMemcpyAsync(norms, h->norms.p, h->M * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
     });

@@ -21,16 +21,24 @@
 //   * the RK4 epilogue of heom_device.hpp.
 // Any coupling (several baths per state, shared baths, uncoupled states) runs through
 // the same code; the term loop is unrolled so a column's table loads and gathers are
-// all in flight together.  The file is compiled twice: for complex128 state and FP64
-// arithmetic, and with HEOM_STAGE_PRECISION defined (stage_kernel_f32.hip) for complex64
-// state and FP32 arithmetic.
+// all in flight together.  The file is compiled three times: for complex128 state and
+// FP64 arithmetic, with HEOM_STAGE_PRECISION defined (stage_kernel_f32.hip) for complex64
+// state and FP32 arithmetic, and with HEOM_STAGE_ADJOINT defined (stage_kernel_adj.hip)
+// for the adjoint operator L^dag in FP64.
+//
+// L^dag (ADJ = true) has the same neighbour structure with the roles of the two edges
+// exchanged: the plus term i c_p sqrt(n_m+1) sigma^{n+e_m} of ADO n is, transposed, a
+// contribution -i c_p sqrt(n_m) mu^{n-e_m} to ADO n, and the minus term
+// c_m sqrt(n_m) sigma^{n-e_m} becomes conj(c_m) sqrt(n_m+1) mu^{n+e_m}.  The host passes
+// -c_p and conj(c_m) in the term list, -H^dag / hbar for H / hbar and the conjugate
+// "same" constants; the kernel exchanges which gather each constant multiplies.
 #include "heom_device.hpp"
 
 namespace heom_hip {
 
 namespace {
 
-template <typename R, int N, int NT, int STAGE, bool REALH>
+template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ>
 __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
                                                                   const c2<R>* __restrict__ gam,
                                                                   const TermT<R>* __restrict__ terms)
@@ -112,84 +120,96 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
         for (int k = 0; k < NT; ++k) {
             const TermT<R> t = terms[e * NT + k];
             const int v = nbv[k].x & 63;
-            const R cp = t.plus_c * lane_value(sq_lane, v + 1);
-            const R sm = lane_value(sq_lane, v);
-            f.x = fma(-cp, xp[k].y, f.x);
-            f.y = fma(cp, xp[k].x, f.y);
-            f = cfma(t.minus_c, mk2<R>(sm * xm[k].x, sm * xm[k].y), f);
+            if (ADJ) {
+                const R cp = t.plus_c * lane_value(sq_lane, v);
+                const R sp = lane_value(sq_lane, v + 1);
+                f.x = fma(-cp, xm[k].y, f.x);
+                f.y = fma(cp, xm[k].x, f.y);
+                f = cfma(t.minus_c, mk2<R>(sp * xp[k].x, sp * xp[k].y), f);
+            } else {
+                const R cp = t.plus_c * lane_value(sq_lane, v + 1);
+                const R sm = lane_value(sq_lane, v);
+                f.x = fma(-cp, xp[k].y, f.x);
+                f.y = fma(cp, xp[k].x, f.y);
+                f = cfma(t.minus_c, mk2<R>(sm * xm[k].x, sm * xm[k].y), f);
+            }
         }
 
         stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, s[c], f);
     }
 }
 
-template <typename R, int N, int NT, bool REALH>
+template <typename R, int N, int NT, bool REALH, bool ADJ>
 void launch_t(int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     const dim3 grid(a.ntiles), block(64 * stage_waves(N));
     switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage<R, N, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 1: hipLaunchKernelGGL((rk4_stage<R, N, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 2: hipLaunchKernelGGL((rk4_stage<R, N, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 3: hipLaunchKernelGGL((rk4_stage<R, N, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    default: hipLaunchKernelGGL((rk4_stage<R, N, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 0: hipLaunchKernelGGL((rk4_stage<R, N, NT, 0, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage<R, N, NT, 1, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage<R, N, NT, 2, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage<R, N, NT, 3, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage<R, N, NT, 4, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
     }
 }
 
-template <typename R, int N, bool REALH>
+template <typename R, int N, bool REALH, bool ADJ>
 void launch_n(int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     switch (nt) {
-    case 2: launch_t<R, N, 2, REALH>(stage, a, el, st); break;
-    case 4: launch_t<R, N, 4, REALH>(stage, a, el, st); break;
-    default: launch_t<R, N, 8, REALH>(stage, a, el, st); break;
+    case 2: launch_t<R, N, 2, REALH, ADJ>(stage, a, el, st); break;
+    case 4: launch_t<R, N, 4, REALH, ADJ>(stage, a, el, st); break;
+    default: launch_t<R, N, 8, REALH, ADJ>(stage, a, el, st); break;
     }
 }
 
-template <typename R, bool REALH>
+template <typename R, bool REALH, bool ADJ>
 void launch_r(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
     switch (N) {
-    case 1: launch_n<R, 1, REALH>(nt, stage, a, el, st); break;
-    case 2: launch_n<R, 2, REALH>(nt, stage, a, el, st); break;
-    case 3: launch_n<R, 3, REALH>(nt, stage, a, el, st); break;
-    case 4: launch_n<R, 4, REALH>(nt, stage, a, el, st); break;
-    case 5: launch_n<R, 5, REALH>(nt, stage, a, el, st); break;
-    case 6: launch_n<R, 6, REALH>(nt, stage, a, el, st); break;
-    case 7: launch_n<R, 7, REALH>(nt, stage, a, el, st); break;
-    default: launch_n<R, 8, REALH>(nt, stage, a, el, st); break;
+    case 1: launch_n<R, 1, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 2: launch_n<R, 2, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 3: launch_n<R, 3, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 4: launch_n<R, 4, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 5: launch_n<R, 5, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 6: launch_n<R, 6, REALH, ADJ>(nt, stage, a, el, st); break;
+    case 7: launch_n<R, 7, REALH, ADJ>(nt, stage, a, el, st); break;
+    default: launch_n<R, 8, REALH, ADJ>(nt, stage, a, el, st); break;
     }
 }
 
-template <typename R>
+template <typename R, bool ADJ>
 void launch_any(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el,
                 hipStream_t s)
 {
     if (real_h)
-        launch_r<R, true>(N, nterm_slots, stage, a, el, s);
+        launch_r<R, true, ADJ>(N, nterm_slots, stage, a, el, s);
     else
-        launch_r<R, false>(N, nterm_slots, stage, a, el, s);
+        launch_r<R, false, ADJ>(N, nterm_slots, stage, a, el, s);
 }
 
 }  // namespace
 
-#ifndef HEOM_STAGE_PRECISION
+#if defined(HEOM_STAGE_ADJOINT)
+void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                          const ElementArgsT<double>& el, hipStream_t s)
+{
+    launch_any<double, true>(N, nterm_slots, real_h, stage, a, el, s);
+}
+#elif !defined(HEOM_STAGE_PRECISION)
 int stage_term_slots(int nterms) { return nterms <= 2 ? 2 : nterms <= 4 ? 4 : 8; }
 
 bool stage_supported(int N, int nterms) { return N >= 1 && N <= kMaxStates && nterms >= 0 && nterms <= 8; }
-#endif
 
-#ifndef HEOM_STAGE_PRECISION
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                   const ElementArgsT<double>& el, hipStream_t s)
 {
-    launch_any<double>(N, nterm_slots, real_h, stage, a, el, s);
+    launch_any<double, false>(N, nterm_slots, real_h, stage, a, el, s);
 }
 #else
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
                   const ElementArgsT<float>& el, hipStream_t s)
 {
-    launch_any<float>(N, nterm_slots, real_h, stage, a, el, s);
+    launch_any<float, false>(N, nterm_slots, real_h, stage, a, el, s);
 }
 #endif
 

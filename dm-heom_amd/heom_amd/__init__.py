@@ -38,6 +38,7 @@ EXPORTED = [
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
+    "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -59,6 +60,12 @@ class Desc(ctypes.Structure):
         ("pf_same_sum", ctypes.c_void_p), ("pf_same", ctypes.c_void_p), ("pf_circ", ctypes.c_void_p),
         ("cbk", ctypes.c_void_p),
     ]
+
+
+class Gradients(ctypes.Structure):
+    """heom_gradients: output pointers (NULL skips an output)."""
+    _fields_ = [("hamiltonian", ctypes.c_void_p), ("pf_same_sum", ctypes.c_void_p), ("pf_same", ctypes.c_void_p),
+                ("pf_circ", ctypes.c_void_p), ("cbk", ctypes.c_void_p), ("state", ctypes.c_void_p)]
 
 
 class TimingInfo(ctypes.Structure):
@@ -110,6 +117,9 @@ def load():
             "heom_stage_kernel": [vp, ctypes.POINTER(ctypes.c_int)],
             "heom_propagate_observe": [vp, dbl, i64, i64, vp, vp, vp],
             "heom_use_graphs": [vp, ctypes.c_int],
+            "heom_adjoint_forward": [vp, dbl, i64, i64, i64, vp, vp],
+            "heom_adjoint_backward": [vp, vp, ctypes.POINTER(Gradients)],
+            "heom_rhs_adjoint": [vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -198,6 +208,8 @@ class Handle:
         self.instance_matrices = [int(d.matrices) for d in descs]
         self.matrices = sum(self.instance_matrices)
         self.width = descs[0].width
+        self.baths = descs[0].baths
+        self.matsubaras = descs[0].matsubaras
         self.precision = descs[0].precision
         self._keep = keepalive
         return self
@@ -363,6 +375,45 @@ class Handle:
         _check(load().heom_stage_kernel(self._h, ctypes.byref(k)))
         return "pipelined" if k.value == HEOM_STAGE_PIPELINED else "tile"
 
+    # adjoint
+    def rhs_adjoint(self) -> np.ndarray:
+        """L^dag applied to the current state (reference layout)."""
+        out = np.empty(self.state_size)
+        _check(load().heom_rhs_adjoint(self._h, out.ctypes.data))
+        return out
+
+    def adjoint_forward(self, h: float, nsteps: int, observe_steps: int, ops, checkpoint_steps: int = 50):
+        """Propagate from the current state recording X_rk = Tr(O_k sigma_0,k(t_r)) (rows,
+        count) and checkpoints for adjoint_backward; the state ends at t = nsteps h."""
+        rows = nsteps // observe_steps + 1
+        o = self._per_instance(ops)
+        out = np.empty(rows * self.count * 2)
+        _check(load().heom_adjoint_forward(self._h, float(h), int(nsteps), int(observe_steps),
+                                           int(checkpoint_steps), o.ctypes.data, out.ctypes.data))
+        out = out.reshape(rows, self.count, 2)
+        return out[..., 0] + 1j * out[..., 1]
+
+    def adjoint_backward(self, weights, state: bool = True) -> dict:
+        """Gradients of J = sum Re(w_rk X_rk) over the recorded sweep: complex arrays
+        (dJ/dRe + i dJ/dIm) for the coefficient arrays -- hamiltonian (count, N, N),
+        pf_same_sum (M), pf_same (B), pf_circ (B), cbk (B*K) -- and 'state', the real
+        gradient with respect to the state the sweep started from."""
+        w = np.asarray(weights, dtype=np.complex128)
+        if w.ndim == 1:
+            w = w[:, None]
+        w = _c128_flat(w)
+        N, B, K = self.states, self.baths, self.matsubaras
+        bufs = dict(hamiltonian=np.zeros(self.count * N * N * 2), pf_same_sum=np.zeros(2 * self.matrices),
+                    pf_same=np.zeros(2 * B), pf_circ=np.zeros(2 * B), cbk=np.zeros(2 * B * K))
+        st = np.zeros(self.state_size) if state else None
+        g = Gradients(*(bufs[k].ctypes.data for k in ("hamiltonian", "pf_same_sum", "pf_same", "pf_circ", "cbk")),
+                      st.ctypes.data if st is not None else None)
+        _check(load().heom_adjoint_backward(self._h, w.ctypes.data, ctypes.byref(g)))
+        out = {k: v[0::2] + 1j * v[1::2] for k, v in bufs.items()}
+        out["hamiltonian"] = out["hamiltonian"].reshape(self.count, N, N)
+        out["state"] = st
+        return out
+
 
 # --------------------------------------------------------------------------------------
 # host side: the config-file instance API (libheom_host.so, include/heom_host.h)
@@ -374,6 +425,7 @@ TASK_LINEAR_ABSORPTION = 1
 HOST_EXPORTED = [
     "heom_host_open", "heom_host_close", "heom_host_last_error", "heom_host_get_info", "heom_host_desc",
     "heom_host_top", "heom_host_cbk_sqrt", "heom_host_mask", "heom_host_dipoles", "heom_host_partitions",
+    "heom_host_set_parameters", "heom_host_get_parameters", "heom_host_parameter_gradient",
 ]
 
 
@@ -410,6 +462,9 @@ def load_host():
             "heom_host_mask": ([vp, vp], ctypes.c_int),
             "heom_host_dipoles": ([vp, ctypes.c_int64, vp, vp], ctypes.c_int),
             "heom_host_partitions": ([ctypes.c_int, ctypes.c_int, vp, ctypes.c_int64], ctypes.c_int64),
+            "heom_host_set_parameters": ([vp, vp, vp, vp, vp], ctypes.c_int),
+            "heom_host_get_parameters": ([vp, vp, vp, vp, vp], ctypes.c_int),
+            "heom_host_parameter_gradient": ([vp, ctypes.POINTER(Gradients), vp, vp, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -454,6 +509,8 @@ class HostInstance:
         _host_check(load_host().heom_host_get_info(h, ctypes.byref(info)))
         self.info = info
         self.states = int(info.states)
+        self.sites = int(info.sites)
+        self.baths = int(info.baths)
         self.matrices = int(info.matrices)
         self.width = int(info.width)
 
@@ -521,6 +578,41 @@ class HostInstance:
         _host_check(load_host().heom_host_dipoles(self._p, int(t), dm.ctypes.data, dp.ctypes.data))
         dm, dp = dm.reshape(n, n, 2), dp.reshape(n, n, 2)
         return dm[..., 0] + 1j * dm[..., 1], dp[..., 0] + 1j * dp[..., 1]
+
+    def set_parameters(self, hamiltonian=None, lam=None, gamma=None, omega=None) -> None:
+        """Replace the site Hamiltonian (sites x sites) and the bath lambda_b, gamma_b =
+        hbar / invnu_b and Omega_b, all in cm^-1 (None keeps); the prefactors are
+        recomputed and the arrays desc() points to hold the new values."""
+        H = _c128_flat(np.asarray(hamiltonian).reshape(self.sites, self.sites)) if hamiltonian is not None else None
+        arrs = [_f64(np.asarray(v).reshape(self.baths)) if v is not None else None for v in (lam, gamma, omega)]
+        _host_check(load_host().heom_host_set_parameters(
+            self._p, H.ctypes.data if H is not None else None,
+            *(a.ctypes.data if a is not None else None for a in arrs)))
+
+    def get_parameters(self) -> dict:
+        """hamiltonian (sites x sites complex), lambda, gamma, omega (B), all in cm^-1."""
+        H = np.zeros(2 * self.sites * self.sites)
+        lam, gam, om = np.zeros(self.baths), np.zeros(self.baths), np.zeros(self.baths)
+        _host_check(load_host().heom_host_get_parameters(self._p, H.ctypes.data, lam.ctypes.data, gam.ctypes.data,
+                                                         om.ctypes.data))
+        H = H.reshape(self.sites, self.sites, 2)
+        return {"hamiltonian": H[..., 0] + 1j * H[..., 1], "lambda": lam, "gamma": gam, "omega": om}
+
+    def parameter_gradient(self, grads: dict, instance: int = 0) -> dict:
+        """Handle.adjoint_backward's gradients (of batch instance `instance`) as gradients
+        with respect to the parameters of set_parameters, per cm^-1: hamiltonian (sites x
+        sites, dJ/dRe + i dJ/dIm per entry), lambda, gamma, omega (B)."""
+        H = _c128_flat(np.asarray(grads["hamiltonian"])[instance] if np.ndim(grads["hamiltonian"]) == 3
+                       else grads["hamiltonian"])
+        arrs = {k: _c128_flat(grads[k]) for k in ("pf_same_sum", "pf_same", "pf_circ", "cbk")}
+        g = Gradients(H.ctypes.data, arrs["pf_same_sum"].ctypes.data, arrs["pf_same"].ctypes.data,
+                      arrs["pf_circ"].ctypes.data, arrs["cbk"].ctypes.data, None)
+        dH = np.zeros(2 * self.sites * self.sites)
+        dl, dg, do = np.zeros(self.baths), np.zeros(self.baths), np.zeros(self.baths)
+        _host_check(load_host().heom_host_parameter_gradient(self._p, ctypes.byref(g), dH.ctypes.data,
+                                                             dl.ctypes.data, dg.ctypes.data, do.ctypes.data))
+        dH = dH.reshape(self.sites, self.sites, 2)
+        return {"hamiltonian": dH[..., 0] + 1j * dH[..., 1], "lambda": dl, "gamma": dg, "omega": do}
 
     def create_handle(self, precision: int = HEOM_PRECISION_FP64) -> Handle:
         """heom_create on the instance, with its initial top ADO loaded."""

@@ -206,7 +206,11 @@ instance::instance(const config& cfg, sites_to_states_mode mode, const hierarchy
     for (std::size_t t = 0; t < C.data.size(); ++t)
         coupling_[t] = (std::int32_t)C.data[t];
     top_.assign((std::size_t)(states_ * states_ * 2), 0.0);
-    precompute(cfg);
+    lambda_ = cfg.baths_lambda();
+    invnu_ = cfg.baths_invnu();
+    omega_ = cfg.baths_uppercase_omega();
+    temperature_ = cfg.baths_temperature();
+    precompute();
 }
 
 // instance::precompute_baths_input (dm-heom/src/heom/instance.cpp:155-262), SI units:
@@ -217,10 +221,10 @@ instance::instance(const config& cfg, sites_to_states_mode mode, const hierarchy
 //     with nu_bk = xi_k/(beta hbar);
 //   Delta_b = 2 lambda_b/(beta hbar^2 nu_b) - sum_k c_bk/(hbar nu_bk);
 //   S_n = -sum_{b,k} n_bk nu_bk (long double).
-void instance::precompute(const config& cfg)
+void instance::precompute()
 {
     const double hbar = constants::h_bar;
-    const double beta = 1.0 / (constants::k_b * cfg.baths_temperature());
+    const double beta = 1.0 / (constants::k_b * temperature_);
     const std::size_t row = (std::size_t)(matsubaras_ - 1) * constants::max_pade;
     const int_t B = baths_, K = matsubaras_;
     pf_same_.assign((std::size_t)(2 * B), 0.0);
@@ -229,8 +233,8 @@ void instance::precompute(const config& cfg)
     cbk_sqrt_.assign((std::size_t)(B * K), 0.0);
 
     for (int_t b = 0; b < B; ++b) {
-        const complex_t lambda_b{cfg.baths_lambda()[(std::size_t)b]};
-        const complex_t nu_b{1.0 / cfg.baths_invnu()[(std::size_t)b], cfg.baths_uppercase_omega()[(std::size_t)b]};
+        const complex_t lambda_b{lambda_[(std::size_t)b]};
+        const complex_t nu_b{1.0 / invnu_[(std::size_t)b], omega_[(std::size_t)b]};
         const complex_t circ = nu_b * lambda_b / hbar;
         complex_t delta = 2.0 * lambda_b / (beta * hbar * hbar * nu_b);
         for (int_t k = 0; k < K; ++k) {
@@ -267,7 +271,7 @@ void instance::precompute(const config& cfg)
     pf_same_sum_.assign((std::size_t)(2 * M), 0.0);
     std::vector<long double> nr((std::size_t)W), ni((std::size_t)W);
     for (int_t b = 0; b < B; ++b) {
-        const complex_t nu_b{1.0 / cfg.baths_invnu()[(std::size_t)b], cfg.baths_uppercase_omega()[(std::size_t)b]};
+        const complex_t nu_b{1.0 / invnu_[(std::size_t)b], omega_[(std::size_t)b]};
         for (int_t k = 0; k < K; ++k) {
             const std::size_t m = (std::size_t)(b * K + k);
             if (k == 0) {
@@ -289,6 +293,99 @@ void instance::precompute(const config& cfg)
         }
         pf_same_sum_[2 * (std::size_t)u] = (double)zr;
         pf_same_sum_[2 * (std::size_t)u + 1] = (double)zi;
+    }
+}
+
+void instance::set_baths(const double* lambda, const double* invnu, const double* omega)
+{
+    for (int_t b = 0; b < baths_; ++b) {
+        if (!(invnu[b] > 0.0))
+            throw std::invalid_argument("set_baths: invnu must be > 0");
+        lambda_[(std::size_t)b] = lambda[b];
+        invnu_[(std::size_t)b] = invnu[b];
+        omega_[(std::size_t)b] = omega[b];
+    }
+    precompute();
+}
+
+// Derivatives of the prefactors of precompute() (no counterpart in the reference).  All
+// of Phi_b, c_bk and Delta_b are linear in lambda_b and holomorphic in nu_b, so with
+// nu = nu_r + i nu_i, d/d nu_r = d/d nu and d/d nu_i = i d/d nu; for a real parameter
+// theta, dJ/d theta = sum over the entries kappa of Re(conj(G_kappa) d kappa/d theta).
+// With a_j = xi_j^2 / (beta hbar)^2:
+//   d c_b0/d nu  = -2 lambda/(beta hbar) sum_j 4 eta_j nu a_j / (a_j - nu^2)^2
+//   d c_bk/d nu  = 4 lambda/(beta hbar)^2 eta_k xi_k (a_k + nu^2) / (a_k - nu^2)^2, k >= 1
+//   d Phi_b/d nu = lambda / hbar
+//   d Delta_b/d nu = -2 lambda/(beta hbar^2 nu^2) - c_b0'/(hbar nu) + c_b0/(hbar nu^2)
+//                    - sum_{k >= 1} c_bk'/(hbar nu_bk)
+//   d S_n/d nu_b = -n_b0 (the Pade poles nu_bk, k >= 1, do not depend on nu_b)
+void instance::bath_gradient(const double* g_pss, const double* g_same, const double* g_circ, const double* g_cbk,
+                             double* d_lambda, double* d_nu_re, double* d_nu_im) const
+{
+    const double hbar = constants::h_bar;
+    const double beta = 1.0 / (constants::k_b * temperature_);
+    const std::size_t row = (std::size_t)(matsubaras_ - 1) * constants::max_pade;
+    const int_t B = baths_, K = matsubaras_, M = matrices_, W = width_;
+    const auto& T = graph_.tuples();
+    auto G = [](const double* a, std::size_t i) { return complex_t(a[2 * i], a[2 * i + 1]); };
+    // Re(conj(g) d)
+    auto dot = [](complex_t g, complex_t d) { return g.real() * d.real() + g.imag() * d.imag(); };
+    for (int_t b = 0; b < B; ++b) {
+        const double lam = lambda_[(std::size_t)b];
+        const complex_t nu{1.0 / invnu_[(std::size_t)b], omega_[(std::size_t)b]};
+        complex_t dl_delta = 2.0 / (beta * hbar * hbar * nu);  // d Delta / d lambda
+        complex_t dn_delta = -2.0 * lam / (beta * hbar * hbar * nu * nu);
+        double dl = 0.0, dnr = 0.0, dni = 0.0;  // dJ/d lambda, dJ/d nu_r, dJ/d nu_i
+        for (int_t k = 0; k < K; ++k) {
+            const std::size_t m = (std::size_t)(b * K + k);
+            complex_t dl_c, dn_c, c, nubk;
+            if (k == 0) {
+                complex_t corr = 0.0, dcorr = 0.0;
+                for (int_t j = 0; j < K; ++j) {
+                    const double eta = (double)constants::eta_pade[row + (std::size_t)j];
+                    const double xi = (double)constants::xi_pade[row + (std::size_t)j];
+                    const double a = xi * xi / (beta * beta * hbar * hbar);
+                    corr += (2.0 * eta * nu * nu) / (a - nu * nu);
+                    dcorr += 4.0 * eta * nu * a / ((a - nu * nu) * (a - nu * nu));
+                }
+                dl_c = 2.0 / (beta * hbar) * (1.0 - corr);
+                c = lam * dl_c;
+                dn_c = -2.0 * lam / (beta * hbar) * dcorr;
+                nubk = nu;
+                // Delta: - c_b0 / (hbar nu)
+                dl_delta -= dl_c / (hbar * nu);
+                dn_delta += -dn_c / (hbar * nu) + c / (hbar * nu * nu);
+            } else {
+                const double eta = (double)constants::eta_pade[row + (std::size_t)k];
+                const double xi = (double)constants::xi_pade[row + (std::size_t)k];
+                const double a = xi * xi / (beta * beta * hbar * hbar);
+                dl_c = 4.0 * nu / (beta * beta * hbar * hbar) * (eta * xi) / (a - nu * nu);
+                dn_c = 4.0 * lam / (beta * beta * hbar * hbar) * (eta * xi) * (a + nu * nu) / ((a - nu * nu) * (a - nu * nu));
+                nubk = xi / (beta * hbar);
+                dl_delta -= dl_c / (hbar * nubk);
+                dn_delta -= dn_c / (hbar * nubk);
+            }
+            const complex_t gc = G(g_cbk, m);
+            dl += dot(gc, dl_c);
+            dnr += dot(gc, dn_c);
+            dni += dot(gc, complex_t(0.0, 1.0) * dn_c);
+        }
+        const complex_t gs = G(g_same, (std::size_t)b), gp = G(g_circ, (std::size_t)b);
+        dl += dot(gs, dl_delta) + dot(gp, nu / hbar);
+        dnr += dot(gs, dn_delta) + dot(gp, complex_t(lam / hbar));
+        dni += dot(gs, complex_t(0.0, 1.0) * dn_delta) + dot(gp, complex_t(0.0, lam / hbar));
+        // S_n = -sum n_bk nu_bk: only the k = 0 mode of bath b moves with nu_b
+        const std::size_t m0 = (std::size_t)(b * K);
+        for (int_t u = 0; u < M; ++u) {
+            const double n = (double)T[(std::size_t)(u * W) + m0];
+            if (n != 0.0) {
+                dnr -= n * g_pss[2 * (std::size_t)u];
+                dni -= n * g_pss[2 * (std::size_t)u + 1];
+            }
+        }
+        d_lambda[b] = dl;
+        d_nu_re[b] = dnr;
+        d_nu_im[b] = dni;
     }
 }
 

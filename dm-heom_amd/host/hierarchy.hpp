@@ -94,6 +94,18 @@ public:
     const hierarchy_graph& graph() const { return graph_; }
 
     void set_hamiltonian(const double* h);
+    // bath parameters in SI units after config::post_process (lambda in J, invnu in s,
+    // Omega in rad/s); set_baths recomputes every prefactor
+    const std::vector<double>& lambda() const { return lambda_; }
+    const std::vector<double>& invnu() const { return invnu_; }
+    const std::vector<double>& omega() const { return omega_; }
+    void set_baths(const double* lambda, const double* invnu, const double* omega);
+    // chain rule from the gradients of J with respect to the prefactor arrays (complex
+    // entries as dJ/dRe + i dJ/dIm; pf_same_sum [2M], pf_same [2B], pf_circ [2B],
+    // cbk [2BK]) to dJ/d lambda_b (per J) and dJ/d Re nu_b, dJ/d Im nu_b (per 1/s),
+    // nu_b = 1/invnu_b + i Omega_b
+    void bath_gradient(const double* g_pss, const double* g_same, const double* g_circ, const double* g_cbk,
+                       double* d_lambda, double* d_nu_re, double* d_nu_im) const;
     // top ADO to load at the start (instance::set_hierarchy_top, instance.cpp:483-495)
     const std::vector<double>& top() const { return top_; }                   // [N*N*2]
     void set_hierarchy_top(const complex_t* top);
@@ -103,12 +115,14 @@ public:
     heom_desc desc() const;
 
 private:
-    void precompute(const config& cfg);
+    void precompute();
 
     sites_to_states_mode mode_;
     const hierarchy_graph& graph_;
     int_t states_, matsubaras_, baths_, baths_per_state_, width_, depth_, matrices_, first_non_plus_;
     std::vector<double> hamiltonian_, pf_same_sum_, pf_same_, pf_circ_, cbk_, cbk_sqrt_, top_;
+    std::vector<double> lambda_, invnu_, omega_;
+    double temperature_ = 0.0;
     std::vector<std::int32_t> coupling_;
 };
 

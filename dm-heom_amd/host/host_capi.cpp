@@ -3,8 +3,10 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "config.hpp"
+#include "constants.hpp"
 #include "heom_host.h"
 #include "hierarchy.hpp"
 #include "solver.hpp"
@@ -163,6 +165,110 @@ int heom_host_dipoles(const heom_host_instance* p, int64_t t, double* d_minus, d
         heom::dipole_matrices(*p->cfg, (std::size_t)t, dm, dp);
         std::memcpy(d_minus, dm.data(), dm.size() * sizeof(heom::complex_t));
         std::memcpy(d_plus, dp.data(), dp.size() * sizeof(heom::complex_t));
+    });
+}
+
+int heom_host_set_parameters(heom_host_instance* p, const double* hamiltonian, const double* lambda,
+                             const double* gamma, const double* omega)
+{
+    return host_guard([&] {
+        need(p);
+        heom::instance& in = *p->inst;
+        const double cm = heom::constants::invcm_to_joule, hbar = heom::constants::h_bar;
+        if (hamiltonian) {
+            const heom::int_t S = p->cfg->system_sites();
+            heom::complex_matrix H(S, S);
+            for (std::size_t e = 0; e < H.data.size(); ++e)
+                H.data[e] = heom::complex_t(hamiltonian[2 * e], hamiltonian[2 * e + 1]) * cm;
+            const heom::complex_matrix Hs = heom::state_hamiltonian(H, in.mode());
+            std::vector<double> flat(Hs.data.size() * 2);
+            for (std::size_t e = 0; e < Hs.data.size(); ++e) {
+                flat[2 * e] = Hs.data[e].real();
+                flat[2 * e + 1] = Hs.data[e].imag();
+            }
+            in.set_hamiltonian(flat.data());
+        }
+        if (lambda || gamma || omega) {
+            const std::size_t B = (std::size_t)in.baths();
+            std::vector<double> l(in.lambda()), iv(in.invnu()), o(in.omega());
+            for (std::size_t b = 0; b < B; ++b) {
+                if (lambda)
+                    l[b] = lambda[b] * cm;
+                if (gamma) {
+                    if (!(gamma[b] > 0.0))
+                        throw std::invalid_argument("gamma must be > 0");
+                    iv[b] = hbar / (gamma[b] * cm);
+                }
+                if (omega)
+                    o[b] = omega[b] * cm / hbar;
+            }
+            in.set_baths(l.data(), iv.data(), o.data());
+        }
+    });
+}
+
+int heom_host_get_parameters(const heom_host_instance* p, double* hamiltonian, double* lambda, double* gamma,
+                             double* omega)
+{
+    return host_guard([&] {
+        need(p);
+        const heom::instance& in = *p->inst;
+        const double cm = heom::constants::invcm_to_joule, hbar = heom::constants::h_bar;
+        if (hamiltonian) {
+            const heom::int_t S = p->cfg->system_sites(), N = in.states(), off = N - S;
+            for (heom::int_t i = 0; i < S; ++i)
+                for (heom::int_t j = 0; j < S; ++j) {
+                    const std::size_t e = (std::size_t)((i + off) * N + (j + off));
+                    hamiltonian[2 * (i * S + j)] = in.hamiltonian()[2 * e] / cm;
+                    hamiltonian[2 * (i * S + j) + 1] = in.hamiltonian()[2 * e + 1] / cm;
+                }
+        }
+        for (std::size_t b = 0; b < (std::size_t)in.baths(); ++b) {
+            if (lambda)
+                lambda[b] = in.lambda()[b] / cm;
+            if (gamma)
+                gamma[b] = hbar / (in.invnu()[b] * cm);
+            if (omega)
+                omega[b] = in.omega()[b] * hbar / cm;
+        }
+    });
+}
+
+int heom_host_parameter_gradient(const heom_host_instance* p, const heom_gradients* g, double* d_hamiltonian,
+                                 double* d_lambda, double* d_gamma, double* d_omega)
+{
+    return host_guard([&] {
+        need(p);
+        if (!g)
+            throw std::invalid_argument("grads is NULL");
+        const heom::instance& in = *p->inst;
+        const double cm = heom::constants::invcm_to_joule, hbar = heom::constants::h_bar;
+        if (d_hamiltonian) {
+            if (!g->hamiltonian)
+                throw std::invalid_argument("grads->hamiltonian is NULL");
+            const heom::int_t S = p->cfg->system_sites(), N = in.states(), off = N - S;
+            for (heom::int_t i = 0; i < S; ++i)
+                for (heom::int_t j = 0; j < S; ++j) {
+                    const std::size_t e = (std::size_t)((i + off) * N + (j + off));
+                    d_hamiltonian[2 * (i * S + j)] = g->hamiltonian[2 * e] * cm;
+                    d_hamiltonian[2 * (i * S + j) + 1] = g->hamiltonian[2 * e + 1] * cm;
+                }
+        }
+        if (d_lambda || d_gamma || d_omega) {
+            if (!g->pf_same_sum || !g->pf_same || !g->pf_circ || !g->cbk)
+                throw std::invalid_argument("bath gradients need pf_same_sum, pf_same, pf_circ and cbk");
+            const std::size_t B = (std::size_t)in.baths();
+            std::vector<double> dl(B), dr(B), di(B);
+            in.bath_gradient(g->pf_same_sum, g->pf_same, g->pf_circ, g->cbk, dl.data(), dr.data(), di.data());
+            for (std::size_t b = 0; b < B; ++b) {
+                if (d_lambda)
+                    d_lambda[b] = dl[b] * cm;
+                if (d_gamma)
+                    d_gamma[b] = dr[b] * cm / hbar;
+                if (d_omega)
+                    d_omega[b] = di[b] * cm / hbar;
+            }
+        }
     });
 }
 

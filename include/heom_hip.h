@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define HEOM_ABI_VERSION 1
+#define HEOM_ABI_VERSION 2
 
 enum heom_status {
     HEOM_OK = 0,
@@ -186,6 +186,41 @@ int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
  * environment variable HEOM_STAGE_KERNEL=tile at heom_create forces the tile kernel. */
 enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1 };
 int heom_stage_kernel(heom_handle* handle, int* kind);
+
+/* ---- adjoint: gradients of observables ------------------------------------------ */
+/* No counterpart in the reference, which has no differentiation; for fitting
+ * Hamiltonians and bath parameters to measured dynamics and spectra.  FP64 handles with
+ * an unfiltered hierarchy (no masked ADOs), N*N <= 64.
+ *
+ * The objective is J = sum_r sum_k Re(w_rk X_rk) with X_rk = Tr(O_k sigma_0,k(t_r)), the
+ * observable O_k on the top ADO of instance k at t_r = r * observe_steps * h,
+ * r = 0 .. nsteps / observe_steps.  heom_adjoint_forward propagates from the current
+ * state like heom_propagate_observe, records X, keeps the state every checkpoint_steps
+ * steps on the device, and leaves the state at t = nsteps * h.  heom_adjoint_backward
+ * takes the weights w (the cotangents of X) and integrates the adjoint of the RK4
+ * trajectory backwards, recomputing each segment from its checkpoint, with the
+ * transposed operator in hand-written kernels; it fills the gradients of J with
+ * respect to the coefficient arrays of heom_desc, each complex entry as
+ * dJ/dRe + i dJ/dIm.  Any pointer may be NULL to skip that output.  Setting the
+ * Hamiltonian or the mask drops the record.  Device memory: (nsteps/checkpoint_steps
+ * + checkpoint_steps + 7) state copies. */
+typedef struct heom_gradients {
+    double* hamiltonian;  /* [count][N*N*2] (per joule) */
+    double* pf_same_sum;  /* [M*2] */
+    double* pf_same;      /* [B*2] (Delta_b), summed over the instances */
+    double* pf_circ;      /* [B*2] (Phi_b), summed over the instances */
+    double* cbk;          /* [B*K*2] (c_bk), summed over the instances */
+    double* state;        /* [M*N*N*2] the state at the start of the forward sweep */
+} heom_gradients;
+
+int heom_adjoint_forward(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
+                         int64_t checkpoint_steps, const double* ops /* [count][N*N*2] */,
+                         double* observed /* [rows][count][2] */);
+int heom_adjoint_backward(heom_handle* handle, const double* weights /* [rows][count][2] */,
+                          heom_gradients* grads);
+/* one evaluation of the transposed operator on the current state, L^dag sigma, into out
+ * [M*N*N*2] (the state is unchanged) */
+int heom_rhs_adjoint(heom_handle* handle, double* out);
 
 #ifdef __cplusplus
 }

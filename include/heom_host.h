@@ -60,6 +60,21 @@ int heom_host_cbk_sqrt(const heom_host_instance* inst, double* out);
 int heom_host_mask(const heom_host_instance* inst, uint8_t* mask);
 /* linear absorption: D- (prefactor applied) and D+ of tensor component t, [N*N*2] each */
 int heom_host_dipoles(const heom_host_instance* inst, int64_t t, double* d_minus, double* d_plus);
+/* Parameters for fitting (no counterpart in the reference, which reads them from the
+ * config only), all in cm^-1: the site Hamiltonian [sites*sites*2] (complex), and per
+ * bath lambda_b, gamma_b = hbar / invnu_b as an energy, and Omega_b [B].  set replaces
+ * the given ones (NULL keeps) and recomputes the prefactors; arrays from heom_host_desc
+ * keep their addresses and hold the new values.  get reads them back. */
+int heom_host_set_parameters(heom_host_instance* inst, const double* hamiltonian, const double* lambda,
+                             const double* gamma, const double* omega);
+int heom_host_get_parameters(const heom_host_instance* inst, double* hamiltonian, double* lambda, double* gamma,
+                             double* omega);
+/* The gradients of heom_adjoint_backward (of this instance's slice for a batch) as
+ * gradients with respect to those parameters, all per cm^-1: d_hamiltonian
+ * [sites*sites*2] (dJ/dRe H_ij + i dJ/dIm H_ij, entries independent), d_lambda,
+ * d_gamma, d_omega [B].  Any output may be NULL. */
+int heom_host_parameter_gradient(const heom_host_instance* inst, const heom_gradients* grads,
+                                 double* d_hamiltonian, double* d_lambda, double* d_gamma, double* d_omega);
 /* integer partitions of n into at most s parts in the reference order; writes
  * min(count, capacity) rows of s entries, returns count (negative on error) */
 int64_t heom_host_partitions(int n, int s, int32_t* out, int64_t capacity);

@@ -21,6 +21,10 @@
  *                        parity unpinned (classical tableau assumed)
  *   norms, mmult_left,   pinned: textbook definitions checked in numpy
  *   trace
+ *   adjoint              no reference counterpart; pinned: <mu, L y> = <L^dag mu, y> on random
+ *                        states against the pinned rhs, every coefficient gradient against
+ *                        central differences of oracle_propagate, and the discrete-adjoint
+ *                        gradient against central differences of whole trajectories
  */
 #include "heom_oracle.h"
 
@@ -803,4 +807,415 @@ void oracle_trace(int states, const double* d, const double* top, double* trace2
     }
     trace2[0] = tr;
     trace2[1] = ti;
+}
+
+/* ======================================================================== */
+/* adjoint: transpose of the RHS and gradients of observables               */
+/* ======================================================================== */
+/* The reference has no differentiation; these functions restate the mathematics of
+ * the discrete adjoint of oracle_rk4_step with oracle_rhs as the operator L.  The RHS is
+ * complex-linear in the state, so the transpose of L as a map of the interleaved real
+ * vector is the complex adjoint L^dag under <a, b> = Re sum conj(a) b. */
+
+static inline const double* in_state(const double* y, int64_t id, int N) { return y + id * N * N * 2; }
+
+/* out = L^dag mu.  Pass 1 transposes the per-ADO parts of oracle_rhs (commutator and
+ * same term); pass 2 scatters every neighbour term of oracle_rhs back onto the ADO it
+ * read, one element plane per thread, with the conjugate coefficient. */
+void oracle_rhs_adjoint(const oracle_instance* inst, const double* mu, double* out)
+{
+    const int N = inst->states;
+    const int K = inst->matsubaras;
+    const int BM = inst->baths_max_per_state;
+    const int W = inst->width;
+    const int64_t M = inst->matrices;
+    const double HB = ORACLE_HBAR;
+    const double* H = inst->hamiltonian;
+
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t id = 0; id < M; ++id) {
+        const double* m = mu + id * N * N * 2;
+        double* o = out + id * N * N * 2;
+        for (int i = 0; i < N; ++i) {
+            for (int j = 0; j < N; ++j) {
+                /* the forward map s -> (i/hbar)(s H - H s) has the adjoint
+                 * mu -> (-i/hbar)(mu H^dag - H^dag mu) */
+                double ar = 0.0, ai = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    const double mr = m[(i * N + k) * 2], mi = m[(i * N + k) * 2 + 1];
+                    const double hr = H[(j * N + k) * 2], hi = -H[(j * N + k) * 2 + 1]; /* conj(H_jk) */
+                    ar += mr * hr - mi * hi;
+                    ai += mr * hi + mi * hr;
+                    const double gr = H[(k * N + i) * 2], gi = -H[(k * N + i) * 2 + 1]; /* conj(H_ki) */
+                    const double nr = m[(k * N + j) * 2], ni = m[(k * N + j) * 2 + 1];
+                    ar -= gr * nr - gi * ni;
+                    ai -= gr * ni + gi * nr;
+                }
+                double o_r = ai / HB, o_i = -ar / HB;
+
+                /* same term: multiplication by conj(S_n - sum Delta) */
+                double sr = inst->pf_same_sum[2 * id], si = inst->pf_same_sum[2 * id + 1];
+                if (i != j) {
+                    for (int b = 0; b < BM; ++b) {
+                        const int bi = inst->coupling[i * BM + b];
+                        if (bi >= 0) {
+                            sr -= inst->pf_same[2 * bi];
+                            si -= inst->pf_same[2 * bi + 1];
+                        }
+                        const int bj = inst->coupling[j * BM + b];
+                        if (bj >= 0) {
+                            sr -= inst->pf_same[2 * bj];
+                            si -= inst->pf_same[2 * bj + 1];
+                        }
+                    }
+                }
+                const double mr = m[(i * N + j) * 2], mi = m[(i * N + j) * 2 + 1];
+                o_r += sr * mr + si * mi;
+                o_i += sr * mi - si * mr;
+                o[(i * N + j) * 2] = o_r;
+                o[(i * N + j) * 2 + 1] = o_i;
+            }
+        }
+    }
+
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < N * N; ++e) {
+        const int i = e / N, j = e % N;
+        for (int64_t id = 0; id < M; ++id) {
+            const double mr = mu[(id * N * N + e) * 2], mi = mu[(id * N * N + e) * 2 + 1];
+            for (int side = 0; side < 2; ++side) {
+                const int s = side == 0 ? i : j;
+                const double sg = side == 0 ? 1.0 : -1.0; /* +i for state i, -i for state j */
+                for (int b = 0; b < BM; ++b) {
+                    const int bs = inst->coupling[s * BM + b];
+                    if (bs < 0)
+                        continue;
+                    for (int m = 0; m < K; ++m) {
+                        const int mm = bs * K + m;
+                        const int t = inst->tuples[id * W + mm];
+                        const double cr = inst->cbk[2 * mm], ci = inst->cbk[2 * mm + 1];
+                        /* plus edge: forward sg * i * a * sigma[pu] added to ADO id */
+                        if (id < inst->first_non_plus) {
+                            const int32_t pu = inst->plus[id * W + mm];
+                            if (!inst->mask || inst->mask[pu]) {
+                                const double a = sqrt((t + 1.0) * hypot(cr, ci) / HB);
+                                /* conj(sg i a) mu = -sg i a mu */
+                                double* op = out + ((int64_t)pu * N * N + e) * 2;
+                                op[0] += sg * a * mi;
+                                op[1] -= sg * a * mr;
+                            }
+                        }
+                        /* minus edge: forward z * sigma[mu] with
+                         * z = (sg i c / hbar + [m == 0] Phi) sqrt(t hbar / |c|) */
+                        if (t > 0) {
+                            const int32_t mu_id = inst->minus[id * W + mm];
+                            if (!inst->mask || inst->mask[mu_id]) {
+                                const double sc = sqrt(t / hypot(cr, ci) * HB);
+                                double zr = -sg * ci / HB, zi = sg * cr / HB;
+                                if (m == 0) {
+                                    zr += inst->pf_circ[2 * bs];
+                                    zi += inst->pf_circ[2 * bs + 1];
+                                }
+                                zr *= sc;
+                                zi *= sc;
+                                double* op = out + ((int64_t)mu_id * N * N + e) * 2;
+                                op[0] += zr * mr + zi * mi;
+                                op[1] += zr * mi - zi * mr;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* g += w * mu conj(B) for a coefficient kappa that enters the RHS as kappa * B */
+static inline void grad_linear(double* g, double w, double mr, double mi, double br, double bi)
+{
+    g[0] += w * (mr * br + mi * bi);
+    g[1] += w * (mi * br - mr * bi);
+}
+
+/* Adds w * d<mu, L y>/d(kappa) for every coefficient array kappa of the instance, each
+ * complex entry as dRe + i dIm: g_h [2 N N], g_pss [2M], g_same [2B], g_circ [2B],
+ * g_cbk [2BK].  Every term of oracle_rhs is differentiated where it is written: the
+ * commutator, S_n and Delta_b enter linearly, Phi_b linearly in the minus term, and c
+ * through c itself and |c| in the plus and minus amplitudes.  Modes with c = 0 are
+ * skipped (their amplitudes are not differentiable). */
+void oracle_adjoint_contract(const oracle_instance* inst, const double* mu, const double* y, double w,
+                             double* g_h, double* g_pss, double* g_same, double* g_circ, double* g_cbk)
+{
+    const int N = inst->states;
+    const int K = inst->matsubaras;
+    const int BM = inst->baths_max_per_state;
+    const int W = inst->width;
+    const int B = W / K;
+    const int64_t M = inst->matrices;
+    const double HB = ORACLE_HBAR;
+
+#pragma omp parallel
+    {
+        double* lh = calloc((size_t)(2 * N * N + 4 * B + 2 * W), sizeof(double));
+        double* ls = lh + 2 * N * N;
+        double* lc = ls + 2 * B;
+        double* lk = lc + 2 * B;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t id = 0; id < M; ++id) {
+            const double* m = mu + id * N * N * 2;
+            const double* s = y + id * N * N * 2;
+            double gp_r = 0.0, gp_i = 0.0;
+            for (int i = 0; i < N; ++i) {
+                for (int j = 0; j < N; ++j) {
+                    const double mr = m[(i * N + j) * 2], mi = m[(i * N + j) * 2 + 1];
+                    const double yr = s[(i * N + j) * 2], yi = s[(i * N + j) * 2 + 1];
+                    /* commutator (i/hbar)(sum_k y_ik H_kj - H_ik y_kj) */
+                    for (int k = 0; k < N; ++k) {
+                        const double ar = s[(i * N + k) * 2], ai = s[(i * N + k) * 2 + 1];
+                        grad_linear(lh + 2 * (k * N + j), w, mr, mi, -ai / HB, ar / HB);
+                        const double br = s[(k * N + j) * 2], bi = s[(k * N + j) * 2 + 1];
+                        grad_linear(lh + 2 * (i * N + k), w, mr, mi, bi / HB, -br / HB);
+                    }
+                    /* S_n y_ij */
+                    gp_r += w * (mr * yr + mi * yi);
+                    gp_i += w * (mi * yr - mr * yi);
+                    /* -Delta_b y_ij for the baths of i and j, i != j */
+                    if (i != j) {
+                        for (int b = 0; b < BM; ++b) {
+                            const int bi = inst->coupling[i * BM + b];
+                            if (bi >= 0)
+                                grad_linear(ls + 2 * bi, w, mr, mi, -yr, -yi);
+                            const int bj = inst->coupling[j * BM + b];
+                            if (bj >= 0)
+                                grad_linear(ls + 2 * bj, w, mr, mi, -yr, -yi);
+                        }
+                    }
+                    for (int side = 0; side < 2; ++side) {
+                        const int st = side == 0 ? i : j;
+                        const double sg = side == 0 ? 1.0 : -1.0;
+                        for (int b = 0; b < BM; ++b) {
+                            const int bs = inst->coupling[st * BM + b];
+                            if (bs < 0)
+                                continue;
+                            for (int k = 0; k < K; ++k) {
+                                const int mm = bs * K + k;
+                                const int t = inst->tuples[id * W + mm];
+                                const double cr = inst->cbk[2 * mm], ci = inst->cbk[2 * mm + 1];
+                                const double ac = hypot(cr, ci);
+                                if (ac == 0.0)
+                                    continue;
+                                if (id < inst->first_non_plus) {
+                                    const int32_t pu = inst->plus[id * W + mm];
+                                    if (!inst->mask || inst->mask[pu]) {
+                                        const double* p = in_state(y, pu, N);
+                                        const double pr = p[(i * N + j) * 2], pi = p[(i * N + j) * 2 + 1];
+                                        /* term sg i a p, a = sqrt((t+1)|c|/hbar), da/dx = a x / (2 |c|^2) */
+                                        const double a = sqrt((t + 1.0) * ac / HB);
+                                        const double da = a / (2.0 * ac * ac);
+                                        /* d term / dx = sg i da x p: Re(conj(mu) i p) = mr (-pi) ... */
+                                        const double q = -(mr * pi) + mi * pr; /* Re(conj(mu) * i p) */
+                                        lk[2 * mm] += w * sg * da * cr * q;
+                                        lk[2 * mm + 1] += w * sg * da * ci * q;
+                                    }
+                                }
+                                if (t > 0) {
+                                    const int32_t mu_id = inst->minus[id * W + mm];
+                                    if (!inst->mask || inst->mask[mu_id]) {
+                                        const double* p = in_state(y, mu_id, N);
+                                        const double pr = p[(i * N + j) * 2], pi = p[(i * N + j) * 2 + 1];
+                                        const double sc = sqrt(t / ac * HB);
+                                        const double dsc = -sc / (2.0 * ac * ac); /* dsc/dx = dsc * x */
+                                        /* z = sg i c / hbar + [k == 0] Phi_b */
+                                        double zr = -sg * ci / HB, zi = sg * cr / HB;
+                                        if (k == 0) {
+                                            zr += inst->pf_circ[2 * bs];
+                                            zi += inst->pf_circ[2 * bs + 1];
+                                            /* Phi_b enters as Phi_b * sc * p */
+                                            grad_linear(lc + 2 * bs, w, mr, mi, sc * pr, sc * pi);
+                                        }
+                                        /* Re(conj(mu) v p) for v = x + i y */
+                                        const double re_p = mr * pr + mi * pi;  /* Re(conj(mu) p) */
+                                        const double re_ip = mi * pr - mr * pi; /* Re(conj(mu) i p) */
+                                        /* d/dcr: sg (i/hbar) sc p + z dsc cr p */
+                                        const double d_r = sg * sc / HB * re_ip + dsc * cr * (zr * re_p + zi * re_ip);
+                                        /* d/dci: -sg/hbar sc p + z dsc ci p */
+                                        const double d_i = -sg * sc / HB * re_p + dsc * ci * (zr * re_p + zi * re_ip);
+                                        lk[2 * mm] += w * d_r;
+                                        lk[2 * mm + 1] += w * d_i;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            g_pss[2 * id] += gp_r;
+            g_pss[2 * id + 1] += gp_i;
+        }
+#pragma omp critical
+        {
+            for (int e = 0; e < 2 * N * N; ++e)
+                g_h[e] += lh[e];
+            for (int e = 0; e < 2 * B; ++e) {
+                g_same[e] += ls[e];
+                g_circ[e] += lc[e];
+            }
+            for (int e = 0; e < 2 * W; ++e)
+                g_cbk[e] += lk[e];
+        }
+        free(lh);
+    }
+}
+
+static void axpy_into(int64_t size, double* out, const double* x, double a, const double* y)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < size; ++e)
+        out[e] = x[e] + a * y[e];
+}
+
+/* Tr(O sigma_0) = sum_ij O_ji sigma_0,ij */
+static void observe_op(int N, const double* op, const double* y, double* x)
+{
+    double xr = 0.0, xi = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const double orr = op[(j * N + i) * 2], oi = op[(j * N + i) * 2 + 1];
+            const double sr = y[(i * N + j) * 2], si = y[(i * N + j) * 2 + 1];
+            xr += orr * sr - oi * si;
+            xi += orr * si + oi * sr;
+        }
+    x[0] = xr;
+    x[1] = xi;
+}
+
+/* J = sum_r Re(w_r X_r), X_r = Tr(O sigma_0(t_r)), t_r = r * observe_steps * h for
+ * r = 0 .. nsteps / observe_steps, over oracle_rk4_step from y0.  Writes X_r into
+ * observed and dJ/d(kappa) (dRe + i dIm) for the coefficient arrays as in
+ * oracle_adjoint_contract, and for the initial state (g_state, may be NULL).
+ *
+ * Discrete adjoint of one RK4 step y' = y + h/6 (k1 + 2 k2 + 2 k3 + k4), k_s = L Y_s:
+ * with lambda = dJ/dy', the adjoint stages A1 = lambda, A2 = lambda + h/2 L^dag A1,
+ * A3 = lambda + h/2 L^dag A2, A4 = lambda + h L^dag A3 give
+ * dJ/dy = lambda + h/6 L^dag (A1 + 2 A2 + 2 A3 + A4) and the coefficient gradient
+ * h/6 <A4, dL Y1> + h/3 <A3, dL Y2> + h/3 <A2, dL Y3> + h/6 <A1, dL Y4>.
+ * The forward states are kept every checkpoint_steps steps and each segment is
+ * recomputed before it is swept backwards.  Returns 0, or -1 on bad arguments or
+ * allocation failure. */
+int oracle_adjoint_gradient(const oracle_instance* inst, double h, int64_t nsteps, int64_t observe_steps,
+                            int64_t checkpoint_steps, const double* op, const double* weights, const double* y0,
+                            double* observed, double* g_h, double* g_pss, double* g_same, double* g_circ,
+                            double* g_cbk, double* g_state)
+{
+    if (nsteps < 0 || observe_steps < 1 || checkpoint_steps < 1)
+        return -1;
+    const int N = inst->states;
+    const int W = inst->width;
+    const int B = W / inst->matsubaras;
+    const int64_t M = inst->matrices;
+    const int64_t size = M * N * N * 2;
+    const int64_t rows = nsteps / observe_steps + 1;
+    const int64_t last = (rows - 1) * observe_steps;
+    const int64_t ck = checkpoint_steps;
+    const int64_t ncp = last / ck + 1;
+
+    double* cps = malloc(sizeof(double) * (size_t)(size * ncp));
+    double* seg = malloc(sizeof(double) * (size_t)(size * ck));
+    double* wk = malloc(sizeof(double) * (size_t)(size * 16));
+    if (!cps || !seg || !wk) {
+        free(cps);
+        free(seg);
+        free(wk);
+        return -1;
+    }
+    double* work = wk;            /* 5 states for oracle_rk4_step */
+    double* y = wk + 5 * size;
+    double* k = wk + 6 * size;
+    double* y2 = wk + 7 * size;
+    double* y3 = wk + 8 * size;
+    double* y4 = wk + 9 * size;
+    double* lam = wk + 10 * size;
+    double* a2 = wk + 11 * size;
+    double* a3 = wk + 12 * size;
+    double* a4 = wk + 13 * size;
+    double* f = wk + 14 * size;
+    double* acc = wk + 15 * size;
+
+    memset(g_h, 0, sizeof(double) * (size_t)(2 * N * N));
+    memset(g_pss, 0, sizeof(double) * (size_t)(2 * M));
+    memset(g_same, 0, sizeof(double) * (size_t)(2 * B));
+    memset(g_circ, 0, sizeof(double) * (size_t)(2 * B));
+    memset(g_cbk, 0, sizeof(double) * (size_t)(2 * W));
+
+    /* forward sweep */
+    memcpy(y, y0, sizeof(double) * (size_t)size);
+    memcpy(cps, y, sizeof(double) * (size_t)size);
+    observe_op(N, op, y, observed);
+    for (int64_t s = 1; s <= last; ++s) {
+        oracle_rk4_step(inst, h, y, work);
+        if (s % observe_steps == 0)
+            observe_op(N, op, y, observed + 2 * (s / observe_steps));
+        if (s % ck == 0 && s / ck < ncp)
+            memcpy(cps + (s / ck) * size, y, sizeof(double) * (size_t)size);
+    }
+
+    /* adjoint source of row r: conj(w_r) O^dag on the top ADO */
+#define ADD_SOURCE(r)                                                                    \
+    do {                                                                                 \
+        const double wr = weights[2 * (r)], wi = weights[2 * (r) + 1];                   \
+        for (int i_ = 0; i_ < N; ++i_)                                                   \
+            for (int j_ = 0; j_ < N; ++j_) {                                             \
+                const double orr = op[(j_ * N + i_) * 2], oi = op[(j_ * N + i_) * 2 + 1]; \
+                lam[(i_ * N + j_) * 2] += wr * orr - wi * oi;                            \
+                lam[(i_ * N + j_) * 2 + 1] -= wr * oi + wi * orr;                        \
+            }                                                                            \
+    } while (0)
+
+    memset(lam, 0, sizeof(double) * (size_t)size);
+    ADD_SOURCE(rows - 1);
+    for (int64_t c = (last - 1) / ck; last > 0 && c >= 0; --c) {
+        const int64_t start = c * ck;
+        const int64_t end = start + ck < last ? start + ck : last;
+        memcpy(seg, cps + c * size, sizeof(double) * (size_t)size);
+        for (int64_t n = start + 1; n < end; ++n) {
+            memcpy(seg + (n - start) * size, seg + (n - start - 1) * size, sizeof(double) * (size_t)size);
+            oracle_rk4_step(inst, h, seg + (n - start) * size, work);
+        }
+        for (int64_t n = end - 1; n >= start; --n) {
+            const double* yn = seg + (n - start) * size;
+            /* forward stages of step n */
+            oracle_rhs(inst, yn, k);
+            axpy_into(size, y2, yn, 0.5 * h, k);
+            oracle_rhs(inst, y2, k);
+            axpy_into(size, y3, yn, 0.5 * h, k);
+            oracle_rhs(inst, y3, k);
+            axpy_into(size, y4, yn, h, k);
+            /* adjoint stages */
+            oracle_rhs_adjoint(inst, lam, f);
+            memcpy(acc, f, sizeof(double) * (size_t)size);
+            axpy_into(size, a2, lam, 0.5 * h, f);
+            oracle_rhs_adjoint(inst, a2, f);
+            axpy_into(size, acc, acc, 2.0, f);
+            axpy_into(size, a3, lam, 0.5 * h, f);
+            oracle_rhs_adjoint(inst, a3, f);
+            axpy_into(size, acc, acc, 2.0, f);
+            axpy_into(size, a4, lam, h, f);
+            oracle_rhs_adjoint(inst, a4, f);
+            axpy_into(size, acc, acc, 1.0, f);
+            oracle_adjoint_contract(inst, a4, yn, h / 6.0, g_h, g_pss, g_same, g_circ, g_cbk);
+            oracle_adjoint_contract(inst, a3, y2, h / 3.0, g_h, g_pss, g_same, g_circ, g_cbk);
+            oracle_adjoint_contract(inst, a2, y3, h / 3.0, g_h, g_pss, g_same, g_circ, g_cbk);
+            oracle_adjoint_contract(inst, lam, y4, h / 6.0, g_h, g_pss, g_same, g_circ, g_cbk);
+            axpy_into(size, lam, lam, h / 6.0, acc);
+            if (n % observe_steps == 0)
+                ADD_SOURCE(n / observe_steps);
+        }
+    }
+#undef ADD_SOURCE
+    if (g_state)
+        memcpy(g_state, lam, sizeof(double) * (size_t)size);
+    free(cps);
+    free(seg);
+    free(wk);
+    return 0;
 }

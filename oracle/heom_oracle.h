@@ -94,6 +94,21 @@ void oracle_mmult_left(int states, int64_t matrices, const double* mx, double* s
 /* Tr(D * top) (include/heom/matrix_trace_observer.hpp:139-158); D NULL = identity */
 void oracle_trace(int states, const double* d, const double* top, double* trace2);
 
+/* ---- adjoint (no reference counterpart: the mathematics of the discrete adjoint) ---- */
+/* out = L^dag mu, the transpose of oracle_rhs as a linear map of the state */
+void oracle_rhs_adjoint(const oracle_instance* inst, const double* mu, double* out);
+/* adds w * d<mu, L y>/d(kappa) (complex entries as dRe + i dIm) for the coefficient
+ * arrays kappa: hamiltonian [2NN], pf_same_sum [2M], pf_same [2B], pf_circ [2B], cbk [2BK] */
+void oracle_adjoint_contract(const oracle_instance* inst, const double* mu, const double* y, double w,
+                             double* g_h, double* g_pss, double* g_same, double* g_circ, double* g_cbk);
+/* J = sum_r Re(w_r Tr(O sigma_0(t_r))) over nsteps RK4 steps from y0 (observations every
+ * observe_steps steps, row 0 at t = 0): observed X_r [2 * rows] and the gradients of J
+ * with respect to the coefficient arrays and (g_state, may be NULL) the initial state,
+ * by the discrete adjoint with checkpoints every checkpoint_steps steps */
+int oracle_adjoint_gradient(const oracle_instance* inst, double h, int64_t nsteps, int64_t observe_steps,
+                            int64_t checkpoint_steps, const double* op, const double* weights, const double* y0,
+                            double* observed, double* g_h, double* g_pss, double* g_same, double* g_circ,
+                            double* g_cbk, double* g_state);
 /* number of OpenMP threads the oracle runs on */
 int oracle_num_threads(void);
 

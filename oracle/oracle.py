@@ -93,6 +93,15 @@ def lib():
         L.oracle_mmult_left.argtypes = [ctypes.c_int, ctypes.c_int64, f64p, f64p]
         L.oracle_trace.restype = None
         L.oracle_trace.argtypes = [ctypes.c_int, ctypes.c_void_p, f64p, f64p]
+        L.oracle_rhs_adjoint.restype = None
+        L.oracle_rhs_adjoint.argtypes = [ctypes.c_void_p, f64p, f64p]
+        L.oracle_adjoint_contract.restype = None
+        L.oracle_adjoint_contract.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_double, f64p, f64p, f64p,
+                                              f64p, f64p]
+        L.oracle_adjoint_gradient.restype = ctypes.c_int
+        L.oracle_adjoint_gradient.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                              ctypes.c_int64, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                                              f64p, ctypes.c_void_p]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_num_threads.argtypes = []
         _lib_handle = L
@@ -569,6 +578,93 @@ class Instance:
         n = self.states
         t = state[: 2 * n * n].reshape(n, n, 2)
         return t[..., 0] + 1j * t[..., 1]
+
+    # -- adjoint -----------------------------------------------------------------------
+    def rhs_adjoint(self, mu: np.ndarray) -> np.ndarray:
+        """L^dag mu, the transpose of rhs() (heom_oracle.c: oracle_rhs_adjoint)."""
+        out = np.empty_like(mu)
+        lib().oracle_rhs_adjoint(self.cstruct(), np.ascontiguousarray(mu), out)
+        return out
+
+    def _grad_arrays(self):
+        N, M, B, W = self.states, self.matrices, self.baths, self.width
+        return (np.zeros(2 * N * N), np.zeros(2 * M), np.zeros(2 * B), np.zeros(2 * B), np.zeros(2 * W))
+
+    @staticmethod
+    def _as_grads(arrays, N):
+        def c(a):
+            return a[0::2] + 1j * a[1::2]
+        h, pss, same, circ, cbk = arrays
+        return {"hamiltonian": c(h).reshape(N, N), "pf_same_sum": c(pss), "pf_same": c(same),
+                "pf_circ": c(circ), "cbk": c(cbk)}
+
+    def adjoint_contract(self, mu: np.ndarray, y: np.ndarray, w: float = 1.0) -> dict:
+        """w * d<mu, L y>/d(kappa) per coefficient array, complex dRe + i dIm."""
+        arrays = self._grad_arrays()
+        lib().oracle_adjoint_contract(self.cstruct(), np.ascontiguousarray(mu), np.ascontiguousarray(y), w,
+                                      *arrays)
+        return self._as_grads(arrays, self.states)
+
+    def adjoint_gradient(self, y0: np.ndarray, h: float, nsteps: int, observe_steps: int, op: np.ndarray,
+                         weights: np.ndarray, checkpoint_steps: int = 50):
+        """J = sum_r Re(w_r Tr(O sigma_0(t_r))): the observables X_r (complex [rows]) and the
+        gradients of J (dict of complex arrays, plus 'state', the real gradient with
+        respect to y0), by the discrete adjoint of the RK4 trajectory."""
+        rows = nsteps // observe_steps + 1
+        N = self.states
+        opa = np.ascontiguousarray(np.stack([np.real(op), np.imag(op)], -1).reshape(-1), dtype=np.float64)
+        w = np.asarray(weights, dtype=np.complex128).reshape(rows)
+        wa = np.ascontiguousarray(np.stack([w.real, w.imag], -1).reshape(-1))
+        observed = np.zeros(2 * rows)
+        arrays = self._grad_arrays()
+        g_state = np.zeros(self.state_size)
+        rc = lib().oracle_adjoint_gradient(self.cstruct(), h, nsteps, observe_steps, checkpoint_steps, opa, wa,
+                                           np.ascontiguousarray(y0, dtype=np.float64), observed, *arrays,
+                                           g_state.ctypes.data)
+        if rc != 0:
+            raise OracleError("oracle_adjoint_gradient failed")
+        g = self._as_grads(arrays, N)
+        g["state"] = g_state
+        return observed[0::2] + 1j * observed[1::2], g
+
+    def precomputed(self, lambda_: np.ndarray, invnu: np.ndarray, omega: np.ndarray) -> dict:
+        """The coefficient arrays oracle_precompute gives for other bath parameters (SI)."""
+        M, B, K = self.matrices, self.baths, self.matsubaras
+        pss, same, circ, cbk, cs = np.zeros(2 * M), np.zeros(2 * B), np.zeros(2 * B), np.zeros(2 * B * K), \
+            np.zeros(B * K)
+        lib().oracle_precompute(B, K, np.ascontiguousarray(lambda_, dtype=np.float64),
+                                np.ascontiguousarray(invnu, dtype=np.float64),
+                                np.ascontiguousarray(omega, dtype=np.float64), self.temperature, M, self.width,
+                                self.tuples, pss, same, circ, cbk, cs)
+
+        def c(a):
+            return a[0::2] + 1j * a[1::2]
+        return {"pf_same_sum": c(pss), "pf_same": c(same), "pf_circ": c(circ), "cbk": c(cbk)}
+
+    def bath_parameter_gradient(self, grads: dict, rel: float = 1e-5) -> np.ndarray:
+        """dJ/d(lambda_b, gamma_b, Omega_b), rows in that order, all per cm^-1, from the
+        coefficient gradients by central differences of oracle_precompute: lambda_b and
+        Omega_b as in the config, gamma_b = hbar / invnu_b as an energy in cm^-1."""
+        B = self.baths
+        out = np.zeros((3, B))
+        keys = ("pf_same_sum", "pf_same", "pf_circ", "cbk")
+        base = np.stack([self.lambda_ / INVCM_TO_JOULE, HBAR / (self.invnu * INVCM_TO_JOULE),
+                         self.omega * HBAR / INVCM_TO_JOULE])
+
+        def si(p):
+            return p[0] * INVCM_TO_JOULE, HBAR / (p[1] * INVCM_TO_JOULE), p[2] * INVCM_TO_JOULE / HBAR
+
+        for b in range(B):
+            for k in range(3):
+                step = rel * max(abs(base[k, b]), 1.0)
+                vals = []
+                for sgn in (1.0, -1.0):
+                    p = base.copy()
+                    p[k, b] += sgn * step
+                    vals.append(self.precomputed(*si(p)))
+                out[k, b] = sum(float(np.sum(np.real(np.conj(grads[key]) * (vals[0][key] - vals[1][key]))))
+                                for key in keys) / (2.0 * step)
+        return out
 
 
 def trace(d: np.ndarray | None, top: np.ndarray) -> complex:

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol(lib, header, listed):
 
 def test_abi_version_and_error_path_without_gpu():
     L = heom_amd.load()
-    assert L.heom_abi_version() == 1
+    assert L.heom_abi_version() == 2
     # a NULL descriptor is rejected before any device call
     h = ctypes.c_void_p()
     rc = L.heom_create(None, ctypes.byref(h))
