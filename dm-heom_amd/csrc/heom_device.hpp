@@ -40,7 +40,7 @@
 namespace heom_hip {
 
 constexpr int kTile = 64;           // ADO granularity of Mp and of the whole-ADO kernels
-constexpr int kMaxStates = 8;       // N handled by the stage kernel
+constexpr int kMaxStates = 8;       // N handled by the tile and pipelined stage kernels
 constexpr int kMaxTupleValue = 62;  // sqrt(n_m + 1) comes from a 64-entry table
 constexpr int64_t kMaxAdos = (int64_t(1) << 26) - 1;  // plus index << 6 fits in 32 bits
 constexpr int kNumXcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin
@@ -265,6 +265,14 @@ void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArg
 // conj(minus_c)) and a.pfss conj(pf_same_sum)
 void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                           const ElementArgsT<double>& el, hipStream_t s);
+
+// ---- large systems (stage_wide.hip): 8 < N <= 32, one wavefront per ADO ---------------
+
+bool wide_supported(int N, int complex_bytes);
+void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
+                       const ElementArgsT<double>& el, hipStream_t s);
+void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                       const ElementArgsT<float>& el, hipStream_t s);
 
 // ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
 

@@ -136,14 +136,20 @@ __global__ __launch_bounds__(64) void observe_tops(const double2* __restrict__ t
                                                    double* __restrict__ pops, double2* __restrict__ traces, int N)
 {
     const int k = blockIdx.x, lane = threadIdx.x, nel = N * N;
-    const int i = lane / N, j = lane % N;
-    const double2 s = lane < nel ? tops[(int64_t)k * nel + lane] : make_double2(0.0, 0.0);
-    if (lane < nel && i == j)
-        pops[(int64_t)k * N + i] = s.x;
+    double re = 0.0, im = 0.0;
+    for (int e = lane; e < nel; e += 64) {
+        const int i = e / N, j = e % N;
+        const double2 s = tops[(int64_t)k * nel + e];
+        if (i == j)
+            pops[(int64_t)k * N + i] = s.x;
+        if (ops) {
+            // Tr(D sigma) = sum_ij D_ji sigma_ij
+            const double2 d = ops[(int64_t)k * nel + j * N + i];
+            re += fma(d.x, s.x, -d.y * s.y);
+            im += fma(d.x, s.y, d.y * s.x);
+        }
+    }
     if (ops) {
-        // Tr(D sigma) = sum_ij D_ji sigma_ij
-        const double2 d = lane < nel ? ops[(int64_t)k * nel + j * N + i] : make_double2(0.0, 0.0);
-        double re = fma(d.x, s.x, -d.y * s.y), im = fma(d.x, s.y, d.y * s.x);
         for (int off = 32; off > 0; off >>= 1) {
             re += __shfl_xor(re, off);
             im += __shfl_xor(im, off);
@@ -222,6 +228,25 @@ __global__ __launch_bounds__(64 * N) void mmult_left_kernel(C* __restrict__ y, c
 #pragma unroll
     for (int j = 0; j < N; ++j)
         y[(int64_t)(i * N + j) * mp + q] = from_d2<C>(out[j]);
+}
+
+// mmult_left for N > 8, out of place: out[e][q] = sum_k mx_ik y[k*N+j][q], thread = (e, q)
+template <typename C>
+__global__ void mmult_left_wide(const C* __restrict__ y, C* __restrict__ out, const double2* __restrict__ mx,
+                                const int32_t* __restrict__ tile_inst, int64_t mp, int N)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nel = N * N;
+    if (t >= (int64_t)nel * mp)
+        return;
+    const int e = (int)(t / mp);
+    const int64_t q = t % mp;
+    const int i = e / N, j = e % N;
+    const double2* m = mx + (tile_inst ? (int64_t)tile_inst[q / kTile] * nel : 0);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k)
+        acc = cfma(m[i * N + k], to_d2(y[(int64_t)(k * N + j) * mp + q]), acc);
+    out[t] = from_d2<C>(acc);
 }
 
 // mmult_left on the FP64 shadow rows: thread = (row r, element i*N+j)
@@ -388,7 +413,9 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     a.xcd_chunk = h->xcd_chunk;
     a.reverse = reverse ? 1 : 0;
     a.c1 = (R)c1;
-    if (h->pipelined)
+    if (h->N > kMaxStates)
+        heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+    else if (h->pipelined)
         heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
     else
         heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
@@ -408,7 +435,9 @@ void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void
     } else {
         const StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, h->pfss.p);
         const ElementArgsT<double> el{h->hs.p, h->gam.p, h->terms.p};
-        if (h->pipelined)
+        if (h->N > kMaxStates)
+            heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+        else if (h->pipelined)
             heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
         else
             heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
@@ -627,7 +656,7 @@ void build_coefficients(heom_handle* h)
             }
         }
     }
-    if (!stage_supported(N, most))
+    if (N <= kMaxStates ? !stage_supported(N, most) : most > 8)
         raise(HEOM_E_UNSUPPORTED, "more than 8 neighbour terms per matrix element");
     h->term_slots = stage_term_slots(most);
     // pad every element's list; padding terms read table column W, which always
@@ -668,8 +697,8 @@ void validate(const heom_desc* d)
         raise(HEOM_E_INVALID, "desc is NULL");
     if (d->states < 1)
         raise(HEOM_E_INVALID, "states must be > 0");
-    if (d->states > kMaxStates)
-        raise(HEOM_E_UNSUPPORTED, "states > 8 not supported by this build");
+    if (d->states > kMaxStates && !wide_supported(d->states, d->precision == HEOM_PRECISION_FP32 ? 8 : 16))
+        raise(HEOM_E_UNSUPPORTED, "states > 32 not supported by this build");
     if (d->matsubaras < 1 || d->baths < 1 || d->baths_max_per_state < 1)
         raise(HEOM_E_INVALID, "matsubaras, baths and baths_max_per_state must be > 0");
     if (d->width != d->baths * d->matsubaras)
@@ -1008,6 +1037,14 @@ void mmult_left_planes(heom_handle* h)
     const dim3 grid(h->ntiles), block(64 * h->N);
     C* y = h->y.as<C>();
     const int32_t* ti = h->count > 1 ? h->tile_inst.p : nullptr;
+    if (h->N > kMaxStates) {
+        const int64_t n = (int64_t)h->nel * h->Mp;
+        hipLaunchKernelGGL(mmult_left_wide<C>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, y, h->ya.as<C>(),
+                           h->mx.p, ti, h->Mp, h->N);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(y, h->ya.p, h->y.bytes(), hipMemcpyDeviceToDevice, h->stream));
+        return;
+    }
     switch (h->N) {
     case 1: hipLaunchKernelGGL((mmult_left_kernel<C, 1>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
     case 2: hipLaunchKernelGGL((mmult_left_kernel<C, 2>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
@@ -1276,8 +1313,6 @@ int heom_propagate_observe(heom_handle* h, double dt, int64_t nsteps, int64_t ob
             raise(HEOM_E_INVALID, "nsteps must be >= 0 and observe_steps >= 1");
         if (!(dt > 0.0))
             raise(HEOM_E_INVALID, "step size must be > 0");
-        if (h->nel > 64)
-            raise(HEOM_E_UNSUPPORTED, "observations need N*N <= 64");
         const int64_t rows = nsteps / observe_steps + 1;
         DevBuf<double> prec;
         DevBuf<double2> trec, opsd;

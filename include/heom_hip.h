@@ -159,7 +159,7 @@ int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observ
  * pops [rows][count][N] and, when ops [count][N*N*2] is given, the traces
  * Tr(ops_k sigma_0) of matrix_trace_observer.hpp:139-158 into traces [rows][count][2]
  * (rows = nsteps / observe_steps + 1).  The reductions run on the device in FP64 (with
- * FP32 storage, on the FP64 top), one wavefront per instance; N*N <= 64. */
+ * FP32 storage, on the FP64 top), one wavefront per instance. */
 int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                            const double* ops, double* pops, double* traces);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel

@@ -1,0 +1,108 @@
+"""Large systems (8 < N <= 32 states, stage_wide.hip): the 24-site workload against the
+oracle, in FP64 (1e-10 relative) and FP32 storage, and the FP32-storage population
+error against FP64 over 1000 fs at depth 4 (20,475 complex64 24 x 24 ADOs)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import as_complex, instance, max_abs_rel, random_state
+
+pytestmark = pytest.mark.gpu
+
+heom_amd = pytest.importorskip("heom_amd")
+FP32 = heom_amd.HEOM_PRECISION_FP32
+
+
+def tier01(inst):
+    return np.nonzero(inst.graph.tuples.sum(axis=1) <= 1)[0]
+
+
+@pytest.mark.parametrize("depth", [1, 2])
+def test_wide_rhs_fp64_matches_oracle(oracle_mod, depth):
+    _, inst = instance("excitonic24_d4", depth=depth)
+    y = random_state(inst, 3)
+    ref = inst.rhs(y)
+    with heom_amd.Handle.from_instance(inst) as h:
+        h.set_state(y)
+        assert np.array_equal(h.get_state(), y)
+        got = h.rhs()
+    err = max_abs_rel(got, ref)
+    print(f"N=24 depth {depth} ({inst.matrices} ADOs): FP64 rhs max rel deviation {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
+def test_wide_rhs_fp32(oracle_mod):
+    _, inst = instance("excitonic24_d4", depth=2)
+    y = random_state(inst, 7)
+    t = tier01(inst)
+    ys = as_complex(y.astype(np.float32).astype(np.float64), inst)
+    ys[t] = as_complex(y, inst)[t]
+    ref = inst.rhs(np.ascontiguousarray(np.stack([ys.real, ys.imag], -1).reshape(-1)))
+    with heom_amd.Handle.from_instance(inst, precision=FP32) as h:
+        h.set_state(y)
+        got = h.rhs()
+    err = max_abs_rel(got, ref)
+    print(f"N=24 FP32 rhs max rel deviation {err:.2e}", flush=True)
+    assert err < 1e-5
+    a, b = as_complex(got, inst), as_complex(ref, inst)
+    assert np.max(np.abs(a[t] - b[t])) / np.max(np.abs(b[t])) < 1e-12
+
+
+def test_wide_propagation_matches_oracle(oracle_mod):
+    cfg, inst = instance("excitonic24_d4", depth=2)
+    y = inst.state_with_top(cfg.rho_init)
+    with heom_amd.Handle.from_instance(inst) as h:
+        h.set_state(y)
+        tops = h.propagate(cfg.step_size, 60, 10)
+        h.set_state(y)
+        pops, tr = h.propagate_observe(cfg.step_size, 60, 10, np.eye(inst.states))
+    ref = inst.propagate(y.copy(), cfg.step_size, 60, 10)
+    err = max_abs_rel(tops, ref)
+    print(f"N=24 60 steps: max rel deviation {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.allclose(pops[:, 0], np.real(np.diagonal(ref, axis1=1, axis2=2)), rtol=0, atol=1e-12)
+    assert np.allclose(tr[:, 0], np.trace(ref, axis1=1, axis2=2), rtol=0, atol=1e-12)
+
+
+def test_wide_batch_and_mmult(oracle_mod):
+    _, a = instance("excitonic24_d4", depth=1)
+    _, b = instance("excitonic24_d4", depth=2)
+    N = a.states
+    rng = np.random.default_rng(4)
+    mx = rng.normal(size=(2, N, N)) + 1j * rng.normal(size=(2, N, N))
+    ys = [random_state(a, 1), random_state(b, 2)]
+    with heom_amd.Handle.from_instances([a, b]) as hb:
+        hb.set_state(np.concatenate(ys))
+        hb.step_rk4(1e-15, 5)
+        sb = hb.get_state()
+        hb.mmult_left(mx)
+        mb = hb.get_state()
+    off = 0
+    for k, inst in enumerate((a, b)):
+        with heom_amd.Handle.from_instance(inst) as h:
+            h.set_state(ys[k])
+            h.step_rk4(1e-15, 5)
+            s1 = h.get_state()
+        n = inst.state_size
+        assert np.array_equal(sb[off:off + n], s1), k
+        want = s1.copy()
+        inst.mmult_left(mx[k], want)
+        assert max_abs_rel(mb[off:off + n], want) < 1e-13
+        off += n
+
+
+def test_wide_fp32_population_error_1000fs(oracle_mod):
+    """The 24-site workload at depth 4: FP32 storage against FP64, 1000 steps of 1 fs."""
+    cfg, inst = instance("excitonic24_d4")
+    assert inst.matrices == 20475
+    y0 = inst.state_with_top(cfg.rho_init)
+    pops = {}
+    for prec in (heom_amd.HEOM_PRECISION_FP64, FP32):
+        with heom_amd.Handle.from_instance(inst, precision=prec) as h:
+            h.set_state(y0)
+            tops = h.propagate(cfg.step_size, cfg.steps, cfg.observe_steps)
+        pops[prec] = np.real(np.diagonal(tops, axis1=1, axis2=2))
+        assert np.max(np.abs(np.real(np.trace(tops, axis1=1, axis2=2)) - 1.0)) < 1e-6
+    err = np.max(np.abs(pops[FP32] - pops[heom_amd.HEOM_PRECISION_FP64]))
+    print(f"24 sites, depth 4: FP32-storage max population error over 1000 fs {err:.2e}", flush=True)
+    assert err < 1e-5
