@@ -2,8 +2,10 @@
 """Throughput of the HEOM hot path on one MI355X: classical RK4 over the full ADO
 hierarchy of the 8-site, 16-mode, depth-8 system (735,471 complex128 8x8 ADOs).
 Beside it: the same workload with FP32 storage (and its population error against FP64
-over 1000 fs), and the batched 512-realisation static-disorder ensemble in FP64 and
-FP32 (ensemble ADO-steps/s and the FP32 population error).
+over 1000 fs), the 24-site system in FP64 and FP32 storage, the batched 512-realisation
+static-disorder ensemble in FP64 and FP32 (ensemble ADO-steps/s and the FP32
+population error), and the adjoint gradient of the 3003-ADO system (time per gradient,
+a central-difference spot check, the CPU oracle's adjoint on a sample).
 
 One "step" is one RK4 step of the whole hierarchy (four fused RHS+RK stage kernels).
 The instance is built by the product's host path (config file -> C++ hierarchy graph
@@ -99,6 +101,129 @@ def precision_run(host, steps_1000fs: int) -> dict:
     return res
 
 
+def wide_precision_run() -> dict:
+    """The 24-site system (24 baths, depth 4, 20,475 ADOs of 24 x 24) in FP64 and with
+    FP32 storage over 1000 fs: rates and the FP32 population error."""
+    import heom_amd
+
+    host = heom_amd.HostInstance(os.path.join(heom_amd.CONFIG_DIR, "excitonic24_d4.cfg"))
+    info = host.info
+    steps = int(round(1e-12 / info.step_size))
+    res, pops = {}, {}
+    for prec, name in ((heom_amd.HEOM_PRECISION_FP64, "fp64"), (heom_amd.HEOM_PRECISION_FP32, "fp32")):
+        h = host.create_handle(precision=prec)
+        h.step_rk4(info.step_size, 2)  # warm-up
+        h.close()
+        h = host.create_handle(precision=prec)
+        tops = h.propagate(info.step_size, steps, 10)
+        t = h.timing().rk4_seconds
+        pops[name] = np.real(np.diagonal(tops, axis1=1, axis2=2))
+        res[name] = {"ms_per_step": t / steps * 1e3, "value": host.matrices * steps / t}
+        h.close()
+    res["max_population_error_vs_fp64"] = float(np.max(np.abs(pops["fp32"] - pops["fp64"])))
+    res["workload"] = f"24 sites, 24 modes, depth {info.depth}, {host.matrices} ADOs, {steps} RK4 steps of 1 fs"
+    res["unit"] = "ADO-RK4-steps/s"
+    host.close()
+    return res
+
+
+def adjoint_run(cpu: bool) -> dict:
+    """Adjoint gradient of J = integral over 1 ps of the population of site 3 (8 sites,
+    8 modes, depth 6, 3003 ADOs, 1000 RK4 steps of 1 fs, checkpoints every 50 steps)
+    with respect to 8 site energies, 28 couplings and 8 (lambda, gamma) pairs: time of
+    one gradient, a spot check against central differences (h = 1e-3 cm^-1) and the
+    CPU oracle's adjoint on a sample of the same workload."""
+    import heom_amd
+
+    cfg_path = os.path.join(heom_amd.CONFIG_DIR, "excitonic8_d6.cfg")
+    host = heom_amd.HostInstance(cfg_path)
+    info = host.info
+    N, nsteps, dt, site = host.states, int(info.steps), float(info.step_size), 3
+    op = np.zeros((N, N))
+    op[site, site] = 1.0
+    w = np.full(nsteps + 1, dt * 1e15)
+    w[0] = w[-1] = 0.5 * dt * 1e15
+
+    def gradient():
+        h = host.create_handle()
+        h.use_graphs(True)
+        t0 = time.perf_counter()
+        X = h.adjoint_forward(dt, nsteps, 1, op, 50)[:, 0]
+        t_fwd = h.timing().last_call_seconds
+        g = h.adjoint_backward(w, state=False)
+        t_bwd = h.timing().last_call_seconds
+        pg = host.parameter_gradient(g)
+        wall = time.perf_counter() - t0
+        h.close()
+        return float(np.sum(w * X.real)), pg, t_fwd, t_bwd, wall
+
+    gradient()
+    J, pg, t_fwd, t_bwd, wall = gradient()
+    p0 = host.get_parameters()
+
+    def objective(H, lam, gam):
+        host.set_parameters(H, lam, gam)
+        h = host.create_handle()
+        _, tr = h.propagate_observe(dt, nsteps, 1, op)
+        h.close()
+        return float(np.sum(w * np.real(tr[:, 0])))
+
+    hstep = 1e-3
+    checks = {}
+    for name in ("eps_3", "J_2_3", "lambda_3", "gamma_3"):
+        vals = []
+        for sgn in (1.0, -1.0):
+            H, lam, gam = p0["hamiltonian"].copy(), p0["lambda"].copy(), p0["gamma"].copy()
+            if name == "eps_3":
+                H[3, 3] += sgn * hstep
+            elif name == "J_2_3":
+                H[2, 3] += sgn * hstep
+                H[3, 2] += sgn * hstep
+            elif name == "lambda_3":
+                lam[3] += sgn * hstep
+            else:
+                gam[3] += sgn * hstep
+            vals.append(objective(H, lam, gam))
+        fd = (vals[0] - vals[1]) / (2 * hstep)
+        adj = {"eps_3": pg["hamiltonian"][3, 3].real, "J_2_3": pg["hamiltonian"][2, 3].real + pg["hamiltonian"][3, 2].real,
+               "lambda_3": pg["lambda"][3], "gamma_3": pg["gamma"][3]}[name]
+        checks[name] = {"adjoint": float(adj), "central_difference": fd, "rel": abs(adj - fd) / abs(fd)}
+    host.set_parameters(p0["hamiltonian"], p0["lambda"], p0["gamma"])
+    host.close()
+    out = {
+        "workload": "8 sites, 8 modes, depth 6 (3003 ADOs), 1000 RK4 steps of 1 fs, checkpoints every 50 steps; "
+                    "J = trapezoidal integral of the site-3 population over 1 ps (fs); gradient w.r.t. 8 eps, "
+                    "28 J_ij, 8 lambda, 8 gamma",
+        "J_fs": J,
+        "parameters": 52,
+        "gradient_seconds": wall,
+        "forward_device_seconds": t_fwd,
+        "backward_device_seconds": t_bwd,
+        "spot_check_h_cm-1": hstep,
+        "max_rel_vs_central_differences": max(c["rel"] for c in checks.values()),
+        "checks": checks,
+    }
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        O.build()
+        cfg = O.parse_config(cfg_path)
+        inst = O.instance_from_config(cfg)
+        y0 = inst.state_with_top(cfg.rho_init)
+        sample = 100
+        ws = np.full(sample + 1, dt * 1e15)
+        t0 = time.perf_counter()
+        inst.adjoint_gradient(y0, dt, sample, 1, op, ws, checkpoint_steps=50)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_oracle"] = {"seconds_per_gradient_estimate": t_cpu * nsteps / sample,
+                             "sample": f"adjoint gradient over the first {sample} of the {nsteps} steps, "
+                                       f"{t_cpu:.1f} s, scaled by {nsteps // sample}",
+                             "cores": int(O.lib().oracle_num_threads())}
+        out["speedup_vs_cpu_oracle"] = out["cpu_oracle"]["seconds_per_gradient_estimate"] / wall
+    return out
+
+
 def ensemble_run(members: int) -> dict:
     """The 512-realisation static-disorder ensemble (7 sites, depths 3-6, populations
     plus xx/yy/zz linear absorption, 2048 steps of 2 fs), batched, FP64 and FP32."""
@@ -114,6 +239,7 @@ def ensemble_run(members: int) -> dict:
         out[name] = {"value": r["ado_steps"] / r["device_seconds"], "device_seconds": r["device_seconds"],
                      "ms_per_step": r["device_seconds"] / r["steps"] * 1e3}
         out["hierarchies"] = ens.members * (1 + ens.ncomp)
+        out[name]["tuning"] = ens.tuning
         out["ados"] = r["ados"]
         ens.close()
     out["max_population_error_fp32_vs_fp64"] = float(
@@ -147,6 +273,7 @@ def main():
     h = host.create_handle()
     M, N = host.matrices, host.states
     dt = info.step_size
+    tuning = h.autotune(steps=3)  # the stage-kernel configuration, timed on this hierarchy
 
     h.step_rk4(dt, args.warmup)
     h.sync()
@@ -213,12 +340,15 @@ def main():
             "avg_launch_ms": per_launch_s * 1e3,
         },
         "checks": {"top_trace_minus_1": trace_err},
+        "tuning": {"kept": next(r for r in tuning if r["chosen"]), "candidates": len(tuning)},
     }
     h.close()
     if not args.no_extras:
         out["precision"] = precision_run(host, int(round(1e-12 / dt)))
+        out["precision"]["sites24"] = wide_precision_run()
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value
+        out["adjoint"] = adjoint_run(not args.no_cpu_baseline)
     host.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg_path, args.cpu_seconds)

@@ -118,6 +118,7 @@ struct heom_handle {
     bool pipelined = false;  // stage_pipelined.hip (persistent) instead of stage_kernel.hip
     int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
     int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
+    int wide_ados = 0;       // N > 8: ADOs per workgroup of the wide kernel (0 = the most that fit)
 
     std::vector<int32_t> coupling;  // [N * BM]
     std::vector<double> cbk, pf_same, pf_circ;

@@ -269,10 +269,13 @@ void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const 
 // ---- large systems (stage_wide.hip): 8 < N <= 32, one wavefront per ADO ---------------
 
 bool wide_supported(int N, int complex_bytes);
+// the largest number of ADOs per workgroup (a power of two <= 16) whose tiles fit in LDS
+int wide_max_ados(int N, int complex_bytes);
+// ados: ADOs per workgroup, 0 = wide_max_ados
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
-                       const ElementArgsT<double>& el, hipStream_t s);
+                       const ElementArgsT<double>& el, int ados, hipStream_t s);
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
-                       const ElementArgsT<float>& el, hipStream_t s);
+                       const ElementArgsT<float>& el, int ados, hipStream_t s);
 
 // ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
 

@@ -414,7 +414,7 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     a.reverse = reverse ? 1 : 0;
     a.c1 = (R)c1;
     if (h->N > kMaxStates)
-        heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+        heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->wide_ados, h->stream);
     else if (h->pipelined)
         heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
     else
@@ -436,7 +436,7 @@ void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void
         const StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, h->pfss.p);
         const ElementArgsT<double> el{h->hs.p, h->gam.p, h->terms.p};
         if (h->N > kMaxStates)
-            heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+            heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->wide_ados, h->stream);
         else if (h->pipelined)
             heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
         else
@@ -1437,7 +1437,109 @@ int heom_stage_kernel(heom_handle* h, int* kind)
         need(h);
         if (!kind)
             raise(HEOM_E_INVALID, "kind is NULL");
-        *kind = h->pipelined ? HEOM_STAGE_PIPELINED : HEOM_STAGE_TILE;
+        *kind = h->N > kMaxStates ? HEOM_STAGE_WIDE : h->pipelined ? HEOM_STAGE_PIPELINED : HEOM_STAGE_TILE;
+    });
+}
+
+// The autotuner: every valid stage-kernel configuration of the handle -- the kernel
+// (tile, pipelined, or for N > 8 the wide kernel), the tiles per XCD chunk of the tile
+// order, the ADOs per workgroup of the wide kernel -- is timed over `steps` RK4 steps
+// (after one untimed step) on the handle's own state, which is restored afterwards; the
+// fastest is kept.  The HIP graph of a step is rebuilt for it.
+int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int32_t capacity, int32_t* count)
+{
+    return guard([&] {
+        need(h);
+        if (steps < 1)
+            raise(HEOM_E_INVALID, "steps must be >= 1");
+        if (capacity > 0 && !results)
+            raise(HEOM_E_INVALID, "results is NULL");
+        struct Cand {
+            int kind, chunk, wgs, ados;
+        };
+        std::vector<Cand> cands;
+        const int nt = h->ntiles;
+        auto chunks = [&](int dflt) {
+            std::vector<int> v{1, 2, 4, 8, 16, 32, 64, dflt};
+            std::vector<int> out;
+            for (int c : v)
+                if (c >= 1 && c <= nt && std::find(out.begin(), out.end(), c) == out.end())
+                    out.push_back(c);
+            return out;
+        };
+        if (h->N > kMaxStates) {
+            for (int T = wide_max_ados(h->N, (int)h->cbytes()); T >= 1; T /= 2)
+                cands.push_back({HEOM_STAGE_WIDE, 1, 0, T});
+        } else {
+            for (int c : chunks(std::max(1, nt / kNumXcd)))
+                cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
+            if (pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes())) {
+                int cus = 0;
+                HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+                const int wgs = (std::max(1, std::min(cus, nt)) + kNumXcd - 1) / kNumXcd * kNumXcd;
+                for (int c : chunks(wgs / kNumXcd))
+                    cands.push_back({HEOM_STAGE_PIPELINED, c, wgs, kTile});
+            }
+        }
+        // the state (and with FP32 storage its FP64 tier shadow) is restored after each trial
+        DevBuf<unsigned char> save, save64;
+        save.alloc(h->y.bytes());
+        HIPC(hipMemcpyAsync(save.p, h->y.p, h->y.bytes(), hipMemcpyDeviceToDevice, h->stream));
+        if (h->fp32()) {
+            save64.alloc(h->y64.bytes());
+            HIPC(hipMemcpyAsync(save64.p, h->y64.p, h->y64.bytes(), hipMemcpyDeviceToDevice, h->stream));
+        }
+        resolve_timing(h);
+        const heom_timing_info timing_kept = h->timing;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        std::vector<double> ms(cands.size());
+        const double dt = 1e-15;
+        auto apply = [&](const Cand& c) {
+            h->pipelined = c.kind == HEOM_STAGE_PIPELINED;
+            h->pipe_wgs = c.kind == HEOM_STAGE_PIPELINED ? c.wgs : h->pipe_wgs;
+            h->xcd_chunk = c.chunk;
+            h->wide_ados = c.kind == HEOM_STAGE_WIDE ? c.ados : 0;
+            drop_graph(h);
+        };
+        try {
+            for (size_t k = 0; k < cands.size(); ++k) {
+                apply(cands[k]);
+                rk4_step_launches(h, dt);
+                HIPC(hipEventRecord(e0, h->stream));
+                for (int s = 0; s < steps; ++s)
+                    rk4_step_launches(h, dt);
+                HIPC(hipEventRecord(e1, h->stream));
+                HIPC(hipEventSynchronize(e1));
+                float t = 0.f;
+                HIPC(hipEventElapsedTime(&t, e0, e1));
+                ms[k] = t / steps;
+                HIPC(hipMemcpyAsync(h->y.p, save.p, h->y.bytes(), hipMemcpyDeviceToDevice, h->stream));
+                if (h->fp32())
+                    HIPC(hipMemcpyAsync(h->y64.p, save64.p, h->y64.bytes(), hipMemcpyDeviceToDevice, h->stream));
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const size_t best = (size_t)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+        apply(cands[best]);
+        HIPC(hipStreamSynchronize(h->stream));
+        h->timing = timing_kept;
+        if (count)
+            *count = (int32_t)cands.size();
+        for (size_t k = 0; k < cands.size() && (int32_t)k < capacity; ++k) {
+            results[k].kernel = cands[k].kind;
+            results[k].xcd_chunk = cands[k].chunk;
+            results[k].workgroups = cands[k].wgs;
+            results[k].ados_per_workgroup = cands[k].ados;
+            results[k].chosen = k == best ? 1 : 0;
+            results[k].ms_per_step = ms[k];
+        }
     });
 }
 

@@ -4,9 +4,9 @@
 //
 // The stage kernels for N <= 8 put one ADO on each lane and a 64-ADO tile of N^2
 // elements in LDS; at N = 24 that tile would need 590 KB.  Here a workgroup owns a
-// tile of T consecutive ADOs (T = 16, or 8 when T + 1 matrices of N^2 elements do not
-// fit the 144 KB LDS budget; T divides the 64-ADO instance granularity) and one
-// wavefront per ADO:
+// tile of T consecutive ADOs (a power of two up to 16 such that T + 1 matrices of N^2
+// elements fit the 144 KB LDS budget -- the most that fit unless the autotuner chose
+// fewer; T divides the 64-ADO instance granularity) and one wavefront per ADO:
 //   * the workgroup loads the tile (plane by plane, T consecutive ADOs per plane
 //     segment) and the instance's H / hbar into LDS; barrier;
 //   * each lane of the ADO's wavefront takes elements e = lane, lane + 64, ... of the
@@ -97,9 +97,10 @@ __global__ __launch_bounds__(1024) void rk4_stage_wide(StageArgsT<R> a, const c2
 }
 
 template <typename R, bool REALH>
-void launch_w(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
+void launch_w(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int ados, hipStream_t st)
 {
-    const int T = wide_tile(N, (int)sizeof(c2<R>));
+    const int fit = wide_tile(N, (int)sizeof(c2<R>));
+    const int T = ados > 0 && ados <= fit && kTile % ados == 0 ? ados : fit;
     const dim3 grid((unsigned)(a.mp / T)), block(64 * T);
     switch (stage) {
     case 0: hipLaunchKernelGGL((rk4_stage_wide<R, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
@@ -117,22 +118,24 @@ bool wide_supported(int N, int complex_bytes)
     return N > kMaxStates && N <= kMaxWideStates && wide_tile(N, complex_bytes) > 0;
 }
 
+int wide_max_ados(int N, int complex_bytes) { return wide_tile(N, complex_bytes); }
+
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
-                       const ElementArgsT<double>& el, hipStream_t s)
+                       const ElementArgsT<double>& el, int ados, hipStream_t s)
 {
     if (real_h)
-        launch_w<double, true>(N, nterm_slots, stage, a, el, s);
+        launch_w<double, true>(N, nterm_slots, stage, a, el, ados, s);
     else
-        launch_w<double, false>(N, nterm_slots, stage, a, el, s);
+        launch_w<double, false>(N, nterm_slots, stage, a, el, ados, s);
 }
 
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
-                       const ElementArgsT<float>& el, hipStream_t s)
+                       const ElementArgsT<float>& el, int ados, hipStream_t s)
 {
     if (real_h)
-        launch_w<float, true>(N, nterm_slots, stage, a, el, s);
+        launch_w<float, true>(N, nterm_slots, stage, a, el, ados, s);
     else
-        launch_w<float, false>(N, nterm_slots, stage, a, el, s);
+        launch_w<float, false>(N, nterm_slots, stage, a, el, ados, s);
 }
 
 }  // namespace heom_hip

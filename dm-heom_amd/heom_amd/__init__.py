@@ -38,10 +38,12 @@ EXPORTED = [
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
-    "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint",
+    "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
+HEOM_STAGE_WIDE = 2
+STAGE_KINDS = {HEOM_STAGE_TILE: "tile", HEOM_STAGE_PIPELINED: "pipelined", HEOM_STAGE_WIDE: "wide"}
 
 
 class HeomError(RuntimeError):
@@ -66,6 +68,12 @@ class Gradients(ctypes.Structure):
     """heom_gradients: output pointers (NULL skips an output)."""
     _fields_ = [("hamiltonian", ctypes.c_void_p), ("pf_same_sum", ctypes.c_void_p), ("pf_same", ctypes.c_void_p),
                 ("pf_circ", ctypes.c_void_p), ("cbk", ctypes.c_void_p), ("state", ctypes.c_void_p)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32), ("workgroups", ctypes.c_int32),
+                ("ados_per_workgroup", ctypes.c_int32), ("chosen", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double)]
 
 
 class TimingInfo(ctypes.Structure):
@@ -120,6 +128,8 @@ def load():
             "heom_adjoint_forward": [vp, dbl, i64, i64, i64, vp, vp],
             "heom_adjoint_backward": [vp, vp, ctypes.POINTER(Gradients)],
             "heom_rhs_adjoint": [vp, vp],
+            "heom_autotune": [vp, ctypes.c_int32, ctypes.POINTER(TuneResult), ctypes.c_int32,
+                              ctypes.POINTER(ctypes.c_int32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -370,10 +380,21 @@ class Handle:
         return b.value
 
     def stage_kernel(self) -> str:
-        """"pipelined" or "tile": the stage kernel this handle runs."""
+        """"pipelined", "tile" or "wide": the stage kernel this handle runs."""
         k = ctypes.c_int()
         _check(load().heom_stage_kernel(self._h, ctypes.byref(k)))
-        return "pipelined" if k.value == HEOM_STAGE_PIPELINED else "tile"
+        return STAGE_KINDS[k.value]
+
+    def autotune(self, steps: int = 3) -> list:
+        """Time every stage-kernel configuration on this handle's state (restored
+        afterwards) and keep the fastest; one dict per candidate."""
+        cap = 64
+        res = (TuneResult * cap)()
+        n = ctypes.c_int32()
+        _check(load().heom_autotune(self._h, int(steps), res, cap, ctypes.byref(n)))
+        return [{"kernel": STAGE_KINDS[r.kernel], "xcd_chunk": r.xcd_chunk, "workgroups": r.workgroups,
+                 "ados_per_workgroup": r.ados_per_workgroup, "chosen": bool(r.chosen), "ms_per_step": r.ms_per_step}
+                for r in res[:min(n.value, cap)]]
 
     # adjoint
     def rhs_adjoint(self) -> np.ndarray:

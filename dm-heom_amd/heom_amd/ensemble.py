@@ -42,7 +42,7 @@ class Ensemble:
     |site_1><site_1| and linear absorption over the xx, yy, zz tensor components."""
 
     def __init__(self, members: int = 512, precision: int = HEOM_PRECISION_FP64, step_fs: float = 2.0,
-                 steps: int = 2048, observe_steps: int = 8, use_graphs: bool = True):
+                 steps: int = 2048, observe_steps: int = 8, use_graphs: bool = True, autotune: bool = True):
         mc = _make_configs()
         H0, dirs = mc.ensemble_base()
         self.members = members
@@ -100,6 +100,12 @@ class Ensemble:
                 self.dminus.append(dm)
                 self.dplus.append(dp)
         self.dminus, self.dplus = np.array(self.dminus), np.array(self.dplus)
+        # stage-kernel configurations timed on the batches themselves (heom_autotune)
+        self.tuning = {}
+        if autotune:
+            self.reset()
+            for name, h in (("populations", self.pop), ("absorption", self.la)):
+                self.tuning[name] = next(r for r in h.autotune(steps=3) if r["chosen"])
 
     def reset(self) -> None:
         """Initial states: |site_1><site_1| (populations) and D+ |g><g| (absorption)."""

@@ -182,10 +182,28 @@ int heom_device_bytes(heom_handle* handle, int64_t* bytes);
 int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
 /* which stage kernel the handle runs (no counterpart in the reference; for tests and
  * benchmarks): HEOM_STAGE_TILE = one workgroup per 64-ADO tile, HEOM_STAGE_PIPELINED =
- * persistent workgroups streaming tiles through LDS (N >= 6 and few bath modes).  The
- * environment variable HEOM_STAGE_KERNEL=tile at heom_create forces the tile kernel. */
-enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1 };
+ * persistent workgroups streaming tiles through LDS (N >= 6 and few bath modes),
+ * HEOM_STAGE_WIDE = one wavefront per ADO for 8 < N <= 32.  The environment variable
+ * HEOM_STAGE_KERNEL=tile at heom_create forces the tile kernel for N <= 8. */
+enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1, HEOM_STAGE_WIDE = 2 };
 int heom_stage_kernel(heom_handle* handle, int* kind);
+
+/* The autotuner (no counterpart in the reference): times every valid configuration of
+ * the stage kernels for this handle -- tile or pipelined kernel and tiles per XCD chunk
+ * of the tile order (N <= 8), ADOs per workgroup (N > 8, HEOM_STAGE_WIDE) -- over `steps`
+ * RK4 steps each on the handle's own state (restored afterwards) and keeps the fastest.
+ * Writes min(count, capacity) results; count receives the number of candidates. */
+typedef struct heom_tune_result {
+    int32_t kernel;              /* enum heom_stage_kind */
+    int32_t xcd_chunk;           /* tiles per XCD chunk of the tile order */
+    int32_t workgroups;          /* pipelined kernel: persistent workgroups, else 0 */
+    int32_t ados_per_workgroup;  /* 64, or the wide kernel's ADOs per workgroup */
+    int32_t chosen;              /* 1 for the configuration kept */
+    int32_t reserved;
+    double ms_per_step;          /* device time of one RK4 step */
+} heom_tune_result;
+int heom_autotune(heom_handle* handle, int32_t steps, heom_tune_result* results, int32_t capacity,
+                  int32_t* count);
 
 /* ---- adjoint: gradients of observables ------------------------------------------ */
 /* No counterpart in the reference, which has no differentiation; for fitting
