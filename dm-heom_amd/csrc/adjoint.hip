@@ -14,7 +14,8 @@
 // with <a, b> = Re sum conj(a) b (the RHS is complex-linear, so its real transpose is
 // L^dag).  The forward sweep keeps the state every checkpoint_steps steps; the backward
 // sweep recomputes each segment's states from its checkpoint, and each step's forward
-// stage vectors from its state, next to the adjoint stages.
+// stage vectors from its state, next to the adjoint stages; one contraction launch per
+// step covers all four stage pairs.
 //
 // The contraction kernel accumulates, per tile of 64 ADOs and in FP64, the sums the
 // gradients of all coefficient arrays of heom_desc follow from:
@@ -41,14 +42,17 @@ using namespace heom_rt;
 
 namespace {
 
+// The four (adjoint stage, forward stage) pairs of one RK4 step: pair p contracts
+// mu = A_{p+1} with Y = Y_{4-p} with weight w_p (dt/6, dt/3, dt/3, dt/6).
+constexpr int kPairs = 4;
 struct ContractArgs {
-    const double2* __restrict__ mu;  // adjoint stage vector (planes)
-    const double2* __restrict__ y;   // forward stage vector (planes)
-    const int2* __restrict__ nb;     // neighbour table
-    double2* __restrict__ part;      // [ntiles][nel * (2 + 2 NT)] per-tile sums
-    double2* __restrict__ gs;        // [Mp] per-ADO sums
+    const double2* __restrict__ mu[kPairs];  // adjoint stage vectors (planes)
+    const double2* __restrict__ y[kPairs];   // forward stage vectors (planes)
+    double w[kPairs];
+    const int2* __restrict__ nb;  // neighbour table
+    double2* __restrict__ part;   // [ntiles][kPairs][nel * (2 + 2 NT)] per-tile sums
+    double2* __restrict__ gs;     // [kPairs][Mp] per-ADO sums
     int64_t mp;
-    double w;                        // weight of this stage pair (dt/6 or dt/3)
 };
 
 __device__ __forceinline__ double wave_sum(double x)
@@ -65,8 +69,16 @@ __device__ __forceinline__ double2 cdot(double2 a, double2 b)
     return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
 }
 
-// Work split as the stage kernel: workgroup = tile of 64 ADOs (lane = ADO), wavefront
-// = 4 columns j of one row i; the tile of both vectors is staged in LDS for T.
+__device__ __forceinline__ void accumulate(double2* o, double w, double re, double im)
+{
+    const double r = wave_sum(re), i = wave_sum(im);
+    if ((threadIdx.x & 63) == 0)
+        *o = make_double2(fma(w, r, o->x), fma(w, i, o->y));
+}
+
+// Work split as the stage kernel: workgroup = tile of 64 ADOs (lane = ADO) and one
+// stage pair (blockIdx.y), wavefront = 4 columns j of one row i; the tile of both
+// vectors is staged in LDS for T.  Each (tile, pair, slot) has one writer.
 template <int N, int NT>
 __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(ContractArgs a,
                                                                         const Term* __restrict__ terms)
@@ -75,6 +87,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
     constexpr int NJW = (N + CW - 1) / CW;
     constexpr int WAVES = N * NJW;
     constexpr int SLOTS = 2 + 2 * NT;
+    constexpr int TB = NT < 4 ? NT : 4;  // terms whose loads are in flight together
     __shared__ double2 sy[N * N * kTile];
     __shared__ double2 sm[N * N * kTile];
     __shared__ double2 red[WAVES * kTile];
@@ -83,17 +96,20 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = wv / NJW;
     const int jb = (wv % NJW) * CW;
-    const int tile = blockIdx.x;
+    const int tile = blockIdx.x, pr = blockIdx.y;
     const int64_t mp = a.mp;
     const int64_t q = (int64_t)tile * kTile + lane;
+    const double2* __restrict__ Y = a.y[pr];
+    const double2* __restrict__ MU = a.mu[pr];
+    const double w = a.w[pr];
 
     double2 ym[CW], mm[CW];
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         const int j = jb + c;
         if (j < N) {
-            ym[c] = a.y[(int64_t)(i * N + j) * mp + q];
-            mm[c] = a.mu[(int64_t)(i * N + j) * mp + q];
+            ym[c] = Y[(int64_t)(i * N + j) * mp + q];
+            mm[c] = MU[(int64_t)(i * N + j) * mp + q];
             sy[(i * N + j) * kTile + lane] = ym[c];
             sm[(i * N + j) * kTile + lane] = mm[c];
         }
@@ -101,15 +117,26 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
     __syncthreads();
 
     double2 dsum = make_double2(0.0, 0.0);
-    double2* out = a.part + (int64_t)tile * (N * N * SLOTS);
+    double2* out = a.part + ((int64_t)tile * kPairs + pr) * (N * N * SLOTS);
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         const int j = jb + c;
         if (j >= N)
             break;
         const int e = i * N + j;
-        const double2* plane = a.y + (int64_t)e * mp;
+        const double2* plane = Y + (int64_t)e * mp;
         double2* o = out + e * SLOTS;
+        // neighbour loads of the first term block go out before the LDS work
+        int2 nbv[TB];
+        double2 xp[TB], xm[TB];
+#pragma unroll
+        for (int k = 0; k < TB; ++k)
+            nbv[k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
+#pragma unroll
+        for (int k = 0; k < TB; ++k) {
+            xp[k] = plane[(uint32_t)nbv[k].x >> 6];
+            xm[k] = plane[(uint32_t)nbv[k].y];
+        }
         double2 tt = make_double2(0.0, 0.0);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -121,23 +148,28 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
         const double2 d = cdot(mm[c], ym[c]);
         dsum.x += d.x;
         dsum.y += d.y;
-        const double r0 = wave_sum(tt.x), r1 = wave_sum(tt.y), r2 = wave_sum(d.x), r3 = wave_sum(d.y);
-        if (lane == 0) {
-            o[0] = make_double2(fma(a.w, r0, o[0].x), fma(a.w, r1, o[0].y));
-            o[1] = make_double2(fma(a.w, r2, o[1].x), fma(a.w, r3, o[1].y));
-        }
-        // one term at a time: gathers, products, reductions
-        for (int k = 0; k < NT; ++k) {
-            const int2 t = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
-            const int v = t.x & 63;
-            const double sp = sqrt((double)(v + 1)), sn = sqrt((double)v);
-            const double2 p = cdot(mm[c], plane[(uint32_t)t.x >> 6]);  // times i sqrt(n+1)
-            const double2 m = cdot(mm[c], plane[(uint32_t)t.y]);
-            const double p0 = wave_sum(-sp * p.y), p1 = wave_sum(sp * p.x);
-            const double m0 = wave_sum(sn * m.x), m1 = wave_sum(sn * m.y);
-            if (lane == 0) {
-                o[2 + k] = make_double2(fma(a.w, p0, o[2 + k].x), fma(a.w, p1, o[2 + k].y));
-                o[2 + NT + k] = make_double2(fma(a.w, m0, o[2 + NT + k].x), fma(a.w, m1, o[2 + NT + k].y));
+        accumulate(o, w, tt.x, tt.y);
+        accumulate(o + 1, w, d.x, d.y);
+#pragma unroll
+        for (int k0 = 0; k0 < NT; k0 += TB) {
+            if (k0 > 0) {
+#pragma unroll
+                for (int k = 0; k < TB; ++k)
+                    nbv[k] = a.nb[(int64_t)terms[e * NT + k0 + k].mode * mp + q];
+#pragma unroll
+                for (int k = 0; k < TB; ++k) {
+                    xp[k] = plane[(uint32_t)nbv[k].x >> 6];
+                    xm[k] = plane[(uint32_t)nbv[k].y];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < TB; ++k) {
+                const int v = nbv[k].x & 63;
+                const double sp = sqrt((double)(v + 1)), sn = sqrt((double)v);
+                const double2 pp = cdot(mm[c], xp[k]);  // times i sqrt(n+1)
+                const double2 m = cdot(mm[c], xm[k]);
+                accumulate(o + 2 + k0 + k, w, -sp * pp.y, sp * pp.x);
+                accumulate(o + 2 + NT + k0 + k, w, sn * m.x, sn * m.y);
             }
         }
     }
@@ -149,24 +181,25 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
             t.x += red[k * kTile + lane].x;
             t.y += red[k * kTile + lane].y;
         }
-        const double2 g = a.gs[q];
-        a.gs[q] = make_double2(fma(a.w, t.x, g.x), fma(a.w, t.y, g.y));
+        double2* g = a.gs + (int64_t)pr * mp + q;
+        *g = make_double2(fma(w, t.x, g->x), fma(w, t.y, g->y));
     }
 }
 
-// out[k][s] = sum of part[t][s] over the tiles t of instance k (tile_inst; null = one)
+// out[k][s] = sum of part[r][s] over the rows r (tile r / per) of instance k (tile_inst;
+// null = one instance)
 __global__ void sum_tiles(const double2* __restrict__ part, const int32_t* __restrict__ tile_inst,
-                          double2* __restrict__ out, int ntiles, int slots, int count)
+                          double2* __restrict__ out, int rows, int per, int slots, int count)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * slots)
         return;
     const int k = t / slots, s = t % slots;
     double2 acc = make_double2(0.0, 0.0);
-    for (int tile = 0; tile < ntiles; ++tile)
-        if (!tile_inst ? k == 0 : tile_inst[tile] == k) {
-            acc.x += part[(int64_t)tile * slots + s].x;
-            acc.y += part[(int64_t)tile * slots + s].y;
+    for (int r = 0; r < rows; ++r)
+        if (!tile_inst ? k == 0 : tile_inst[r / per] == k) {
+            acc.x += part[(int64_t)r * slots + s].x;
+            acc.y += part[(int64_t)r * slots + s].y;
         }
     out[t] = acc;
 }
@@ -205,7 +238,7 @@ __global__ void add_tops(double2* __restrict__ lam, const double2* __restrict__ 
 template <int N, int NT>
 void launch_contract_t(const ContractArgs& a, const Term* terms, int ntiles, hipStream_t st)
 {
-    hipLaunchKernelGGL((adjoint_contract<N, NT>), dim3(ntiles), dim3(64 * stage_waves(N)), 0, st, a, terms);
+    hipLaunchKernelGGL((adjoint_contract<N, NT>), dim3(ntiles, kPairs), dim3(64 * stage_waves(N)), 0, st, a, terms);
 }
 
 template <int N>
@@ -218,17 +251,21 @@ void launch_contract_n(int nt, const ContractArgs& a, const Term* terms, int nti
     }
 }
 
-void launch_contract(heom_handle* h, const void* mu, const void* y, double w)
+// the gradient contributions of one RK4 step: pairs (A1, Y4), (A2, Y3), (A3, Y2), (A4, Y1)
+void launch_contract(heom_handle* h, const void* const* mu, const void* const* y, double dt)
 {
     AdjointState& A = *h->adj;
     ContractArgs a{};
-    a.mu = reinterpret_cast<const double2*>(mu);
-    a.y = reinterpret_cast<const double2*>(y);
+    const double w[kPairs] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+    for (int p = 0; p < kPairs; ++p) {
+        a.mu[p] = reinterpret_cast<const double2*>(mu[p]);
+        a.y[p] = reinterpret_cast<const double2*>(y[p]);
+        a.w[p] = w[p];
+    }
     a.nb = h->nb.p;
     a.part = A.part.p;
     a.gs = A.gs.p;
     a.mp = h->Mp;
-    a.w = w;
     const int nt = h->term_slots, nb = h->ntiles;
     switch (h->N) {
     case 1: launch_contract_n<1>(nt, a, h->terms.p, nb, h->stream); break;
@@ -465,14 +502,14 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
             }
 
         A.seg.alloc(pb * (size_t)std::min<int64_t>(ck, std::max<int64_t>(A.last, 1)));
-        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b})
+        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.c})
             b->alloc(pb);
-        A.part.alloc((size_t)h->ntiles * slots);
-        A.gs.alloc(h->Mp);
+        A.part.alloc((size_t)h->ntiles * kPairs * slots);
+        A.gs.alloc((size_t)kPairs * h->Mp);
         A.src.alloc(src.size());
         HIPC(hipMemcpyAsync(A.src.p, src.data(), src.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
         // padding slots of the state planes must read as zero
-        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.seg})
+        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.c, &A.seg})
             HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
         HIPC(hipMemsetAsync(A.part.p, 0, A.part.bytes(), h->stream));
         HIPC(hipMemsetAsync(A.gs.p, 0, A.gs.bytes(), h->stream));
@@ -491,14 +528,15 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
                 for (int64_t n = end - 1; n >= start; --n) {
                     unsigned char* yn = slot(A.seg, pb, n - start);
                     rk4_step_into(h, yn, nullptr, dt);  // Y2, Y3, Y4 of step n
+                    // adjoint stages A2 (a), A3 (b), A4 (c) from A1 = lambda, then all four
+                    // stage pairs in one contraction, then dJ/dy_n into lambda
                     launch_stage_buffers(h, 1, A.lam.p, A.lam.p, h->pbuf.p, A.a.p, 0.5 * dt, false, true);
-                    launch_contract(h, A.lam.p, A.f4.p, dt / 6.0);
                     launch_stage_buffers(h, 2, A.a.p, A.lam.p, h->pbuf.p, A.b.p, 0.5 * dt, false, true);
-                    launch_contract(h, A.a.p, A.f3.p, dt / 3.0);
-                    launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.a.p, dt, false, true);
-                    launch_contract(h, A.b.p, A.f2.p, dt / 3.0);
-                    launch_contract(h, A.a.p, yn, dt / 6.0);
-                    launch_stage_buffers(h, 4, A.a.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
+                    launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.c.p, dt, false, true);
+                    const void* mus[kPairs] = {A.lam.p, A.a.p, A.b.p, A.c.p};
+                    const void* ys[kPairs] = {A.f4.p, A.f3.p, A.f2.p, yn};
+                    launch_contract(h, mus, ys, dt);
+                    launch_stage_buffers(h, 4, A.c.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
                     if (n % A.observe_steps == 0)
                         add_source(h, nonzero, n / A.observe_steps);
                 }
@@ -514,13 +552,14 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
         DevBuf<double2> red;
         red.alloc((size_t)count * slots);
         hipLaunchKernelGGL(sum_tiles, dim3(blocks_for((int64_t)count * slots, 256)), dim3(256), 0, h->stream,
-                           A.part.p, count > 1 ? h->tile_inst.p : nullptr, red.p, h->ntiles, slots, count);
+                           A.part.p, count > 1 ? h->tile_inst.p : nullptr, red.p, h->ntiles * kPairs, kPairs,
+                           slots, count);
         HIPC(hipGetLastError());
-        std::vector<double2> sums((size_t)count * slots), gs(h->Mp);
+        std::vector<double2> sums((size_t)count * slots), gs4((size_t)kPairs * h->Mp), gs(h->Mp);
         std::vector<int32_t> q2u(h->Mp);
         std::vector<Term> terms((size_t)nel * NT);
         HIPC(hipMemcpyAsync(sums.data(), red.p, red.bytes(), hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipMemcpyAsync(gs.data(), A.gs.p, A.gs.bytes(), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(gs4.data(), A.gs.p, A.gs.bytes(), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipMemcpyAsync(q2u.data(), h->q2u.p, q2u.size() * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipMemcpyAsync(terms.data(), h->terms.p, terms.size() * sizeof(Term), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
@@ -529,6 +568,14 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         h->timing.last_call_seconds = ms * 1e-3;
+        for (int64_t q = 0; q < h->Mp; ++q) {
+            double2 t = make_double2(0.0, 0.0);
+            for (int p = 0; p < kPairs; ++p) {
+                t.x += gs4[(size_t)p * h->Mp + q].x;
+                t.y += gs4[(size_t)p * h->Mp + q].y;
+            }
+            gs[q] = t;
+        }
 
         using cd = std::complex<double>;
         auto at = [&](int k, int e, int s) {

@@ -87,9 +87,9 @@ struct AdjointState {
 
     DevBuf<unsigned char> seg;                   // [checkpoint_steps] states y_n of a segment
     DevBuf<unsigned char> f2, f3, f4;            // forward stage vectors Y2, Y3, Y4
-    DevBuf<unsigned char> lam, a, b;             // adjoint state and stage vectors
-    DevBuf<double2> part;                        // [ntiles][slots] per-tile gradient sums
-    DevBuf<double2> gs;                          // [Mp] per-ADO d J / d pf_same_sum
+    DevBuf<unsigned char> lam, a, b, c;          // adjoint state and stage vectors A2, A3, A4
+    DevBuf<double2> part;                        // [ntiles][4 stage pairs][slots] per-tile gradient sums
+    DevBuf<double2> gs;                          // [4 stage pairs][Mp] per-ADO d J / d pf_same_sum
     DevBuf<double2> src;                         // [rows][count][nel] adjoint sources conj(w_r) O^dag
 };
 

@@ -69,6 +69,24 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
             s[c] = mk2<R>(R(0), R(0));
         }
     }
+    // with few terms per element, the table entries and the epilogue operands of all
+    // columns are requested before the barrier: they do not depend on the LDS tile, so
+    // their latency overlaps the tile's own loads
+    constexpr bool kHoist = NT <= 2;
+    int2 nbh[kHoist ? CW : 1][kHoist ? NT : 1];
+    C eph[kHoist ? CW : 1];
+    if (kHoist) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int j = jb + c;
+            if (j < N) {
+#pragma unroll
+                for (int k = 0; k < NT; ++k)
+                    nbh[c][k] = a.nb[(int64_t)terms[(i * N + j) * NT + k].mode * mp + q];
+                eph[c] = stage_operand<STAGE, R>(a, (int64_t)(i * N + j) * mp + q);
+            }
+        }
+    }
     const C S = a.pfss[q];
     __syncthreads();
 
@@ -90,7 +108,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
         int2 nbv[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k)
-            nbv[k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
+            nbv[k] = kHoist ? nbh[kHoist ? c : 0][kHoist ? k : 0] : a.nb[(int64_t)terms[e * NT + k].mode * mp + q];
         C xp[NT], xm[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
@@ -135,7 +153,13 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
             }
         }
 
-        stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, s[c], f);
+        if (kHoist) {
+            C r0, r1 = mk2<R>(R(0), R(0));
+            stage_values<STAGE, R>(a.c1, s[c], f, eph[kHoist ? c : 0], r0, r1);
+            stage_store<STAGE, R>(a, (int64_t)e * mp + q, r0, r1);
+        } else {
+            stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, s[c], f);
+        }
     }
 }
 
