@@ -115,10 +115,12 @@ def wide_precision_run() -> dict:
         h.step_rk4(info.step_size, 2)  # warm-up
         h.close()
         h = host.create_handle(precision=prec)
+        kept = next(r for r in h.autotune(steps=3) if r["chosen"])
         tops = h.propagate(info.step_size, steps, 10)
         t = h.timing().rk4_seconds
         pops[name] = np.real(np.diagonal(tops, axis1=1, axis2=2))
-        res[name] = {"ms_per_step": t / steps * 1e3, "value": host.matrices * steps / t}
+        res[name] = {"ms_per_step": t / steps * 1e3, "value": host.matrices * steps / t,
+                     "ados_per_workgroup": kept["ados_per_workgroup"]}
         h.close()
     res["max_population_error_vs_fp64"] = float(np.max(np.abs(pops["fp32"] - pops["fp64"])))
     res["workload"] = f"24 sites, 24 modes, depth {info.depth}, {host.matrices} ADOs, {steps} RK4 steps of 1 fs"

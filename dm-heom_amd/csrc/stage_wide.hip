@@ -4,18 +4,11 @@
 //
 // The stage kernels for N <= 8 put one ADO on each lane and a 64-ADO tile of N^2
 // elements in LDS; at N = 24 that tile would need 590 KB.  Here a workgroup owns a
-// tile of T consecutive ADOs (a power of two up to 16 such that T + 1 matrices of N^2
-// elements fit the 144 KB LDS budget -- the most that fit unless the autotuner chose
-// fewer; T divides the 64-ADO instance granularity) and one wavefront per ADO:
-//   * the workgroup loads the tile (plane by plane, T consecutive ADOs per plane
-//     segment) and the instance's H / hbar into LDS; barrier;
-//   * each lane of the ADO's wavefront takes elements e = lane, lane + 64, ... of the
-//     ADO: the commutator (i/hbar)(sigma H - H sigma)_ij from LDS (lanes of one pass
-//     share rows and read consecutive columns), the "same" term, and the element's
-//     neighbour terms (table entry and gathers of the neighbours' element e);
-//   * the RK4 epilogue of heom_device.hpp per element.
-// The per-ADO constants (pf_same_sum) are wavefront-uniform; the per-element
-// constants (gamma_ij, the term list) are per lane.
+// tile of T consecutive ADOs (a power of two up to 8 such that H and T matrices fit
+// the 144 KB LDS budget -- the most that fit unless the autotuner chose fewer; T
+// divides the 64-ADO instance granularity), and works in the three phases described
+// at the kernel: the commutator per ADO wavefront from LDS, everything that touches
+// global memory plane by plane across the tile's consecutive ADOs.
 #include "heom_device.hpp"
 
 namespace heom_hip {
@@ -25,74 +18,196 @@ namespace {
 constexpr int kWideLdsBytes = 147456;  // 144 KB of the 160 KB LDS
 constexpr int kMaxWideStates = 32;
 
-// ADOs per workgroup for N states with `cbytes` per complex element, 0 if none fits
+// LDS row stride (complex elements) of one N x N matrix: one pad element breaks the
+// bank pattern of the 8 rows a wavefront reads at once
+__host__ __device__ constexpr int wide_row(int N) { return N + 1; }
+
+// at most 8 wavefronts per workgroup, so that a lane may hold up to 256 VGPRs (the
+// block's accumulators, operands and in-flight loads)
+constexpr int kWideThreads = 512;
+
+// ADOs per workgroup for N states with `cbytes` per complex element, 0 if none fits:
+// the LDS holds H and the tile (later overwritten by the commutator-plus-same-term values)
 int wide_tile(int N, int cbytes)
 {
-    const int per = N * N * cbytes;
-    for (int T = 16; T >= 1; T /= 2)
+    const int per = N * wide_row(N) * cbytes;
+    for (int T = kWideThreads / 64; T >= 1; T /= 2)
         if ((T + 1) * per <= kWideLdsBytes)
             return T;
     return 0;
 }
 
-template <typename R, int STAGE, bool REALH>
-__global__ __launch_bounds__(1024) void rk4_stage_wide(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
-                                                       const c2<R>* __restrict__ gam,
-                                                       const TermT<R>* __restrict__ terms, int N, int nt, int T)
+// Three phases per workgroup (T ADOs):
+//   A  the tile sigma and H / hbar into LDS, plane by plane (T consecutive ADOs of one
+//      element per T consecutive threads);
+//   B  wavefront = ADO: lane l owns the B x B block of elements (i, j) with
+//      i = (l / 8) * B + r, j = (l % 8) * B + c (B = ceil(N / 8)); the commutator of the
+//      block needs B rows of sigma and H and B columns of both per k, 4B LDS reads for
+//      B^2 complex products; after a barrier, commutator plus same term overwrite the
+//      tile in LDS;
+//   C  back to the plane-by-plane mapping: each thread takes (ADO, element) pairs, adds
+//      the element's neighbour terms (table entry and gathers) to the LDS value and runs
+//      the RK4 epilogue with the own element re-read from global memory (still in L2),
+//      so that loads of the stage operand and all stores move T consecutive ADOs of
+//      one plane together.  LDS: H and one tile, so that several workgroups share a CU.
+template <typename R, int B, int NT, int STAGE, bool REALH>
+__global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
+                                                               const c2<R>* __restrict__ gam,
+                                                               const TermT<R>* __restrict__ terms, int N, int T)
 {
     using C = c2<R>;
     __shared__ C lds[kWideLdsBytes / sizeof(C)];
-    const int nel = N * N;
-    C* hl = lds;          // [nel] H / hbar
-    C* sig = lds + nel;   // [T][nel]
+    const int nel = N * N, NP = wide_row(N), mat = N * NP;
+    C* hl = lds;         // [N][NP] H / hbar
+    C* sig = lds + mat;  // [T][N][NP] the tile, then commutator + same term
 
     const int64_t mp = a.mp;
     const int64_t q0 = (int64_t)blockIdx.x * T;
     const C* hs = hs_all + (a.tile_inst ? (int64_t)a.tile_inst[q0 / kTile] * nel : 0);
     for (int idx = threadIdx.x; idx < nel; idx += blockDim.x)
-        hl[idx] = hs[idx];
+        hl[(idx / N) * NP + idx % N] = hs[idx];
     for (int idx = threadIdx.x; idx < T * nel; idx += blockDim.x) {
         const int ad = idx % T, e = idx / T;
-        sig[ad * nel + e] = a.yin[(int64_t)e * mp + q0 + ad];
+        sig[ad * mat + (e / N) * NP + e % N] = a.yin[(int64_t)e * mp + q0 + ad];
     }
     __syncthreads();
 
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int64_t q = q0 + wv;
-    const C S = a.pfss[q];
-    const C* s = sig + wv * nel;
-    for (int e = lane; e < nel; e += 64) {
-        const int i = e / N, j = e - i * N;
-        C cm = mk2<R>(R(0), R(0));
+    // phase B
+    {
+        const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int lane = threadIdx.x & 63;
+        const int i0 = (lane >> 3) * B, j0 = (lane & 7) * B;
+        const C S = a.pfss[q0 + wv];
+        const C* s = sig + wv * mat;
+        int ir[B], jc[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            ir[r] = i0 + r < N ? i0 + r : 0;  // rows beyond N read row 0 and are not kept
+            jc[r] = j0 + r < N ? j0 + r : 0;
+        }
+        C cm[B][B];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                cm[r][c] = mk2<R>(R(0), R(0));
+#pragma unroll 4
         for (int k = 0; k < N; ++k) {
-            const C sik = s[i * N + k], skj = s[k * N + j];
-            const C hkj = hl[k * N + j], hik = hl[i * N + k];
-            if (REALH) {
-                cm.x = fma(sik.x, hkj.x, fma(-hik.x, skj.x, cm.x));
-                cm.y = fma(sik.y, hkj.x, fma(-hik.x, skj.y, cm.y));
-            } else {
-                cm = cfma(sik, hkj, cm);
-                cm = cfma(mk2<R>(-hik.x, -hik.y), skj, cm);
+            C sik[B], hik[B], hkj[B], skj[B];
+#pragma unroll
+            for (int r = 0; r < B; ++r) {
+                sik[r] = s[ir[r] * NP + k];
+                hik[r] = hl[ir[r] * NP + k];
+                hkj[r] = hl[k * NP + jc[r]];
+                skj[r] = s[k * NP + jc[r]];
+            }
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+#pragma unroll
+                for (int c = 0; c < B; ++c) {
+                    if (REALH) {
+                        cm[r][c].x = fma(sik[r].x, hkj[c].x, fma(-hik[r].x, skj[c].x, cm[r][c].x));
+                        cm[r][c].y = fma(sik[r].y, hkj[c].x, fma(-hik[r].x, skj[c].y, cm[r][c].y));
+                    } else {
+                        cm[r][c] = cfma(sik[r], hkj[c], cm[r][c]);
+                        cm[r][c] = cfma(mk2<R>(-hik[r].x, -hik[r].y), skj[c], cm[r][c]);
+                    }
+                }
+        }
+        C fv[B][B];
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                const int i = ir[r], j = jc[c];
+                const C g = gam[i * N + j];
+                fv[r][c] = cfma(mk2<R>(S.x + g.x, S.y + g.y), s[i * NP + j], mk2<R>(-cm[r][c].y, cm[r][c].x));
+            }
+        __syncthreads();  // every wavefront is done reading the tile
+        C* fo = sig + wv * mat;
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c)
+                if (i0 + r < N && j0 + c < N)
+                    fo[(i0 + r) * NP + j0 + c] = fv[r][c];
+    }
+    __syncthreads();
+
+    // phase C, two (ADO, element) pairs per thread and pass so that their loads overlap
+    const int total = T * nel, stride = blockDim.x;
+    for (int base = threadIdx.x; base < total; base += 2 * stride) {
+        int2 nbv[2][NT];
+        C ep[2], own[2], xp[2][NT], xm[2][NT];
+        int64_t o[2];
+        int e2[2], lo[2];
+        bool live[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int idx = base + u * stride;
+            live[u] = idx < total;
+            const int id = live[u] ? idx : base;  // a dead second pair repeats the first's loads
+            const int ad = id % T, e = id / T;
+            e2[u] = e;
+            lo[u] = ad * mat + (e / N) * NP + e % N;
+            o[u] = (int64_t)e * mp + q0 + ad;
+#pragma unroll
+            for (int k = 0; k < NT; ++k)
+                nbv[u][k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q0 + ad];
+            ep[u] = stage_operand<STAGE, R>(a, o[u]);
+            own[u] = a.yin[o[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const C* plane = a.yin + (int64_t)e2[u] * mp;
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                xp[u][k] = plane[(uint32_t)nbv[u][k].x >> 6];
+                xm[u][k] = plane[(uint32_t)nbv[u][k].y];
             }
         }
-        const C g = gam[e];
-        const C se = s[e];
-        C f = cfma(mk2<R>(S.x + g.x, S.y + g.y), se, mk2<R>(-cm.y, cm.x));
-        const C* plane = a.yin + (int64_t)e * mp;
-        for (int k = 0; k < nt; ++k) {
-            const TermT<R> t = terms[e * nt + k];
-            const int2 nbv = a.nb[(int64_t)t.mode * mp + q];
-            const int v = nbv.x & 63;
-            const C xp = plane[(uint32_t)nbv.x >> 6];
-            const C xm = plane[(uint32_t)nbv.y];
-            const R cp = t.plus_c * sqrt((R)(v + 1));
-            const R sm = sqrt((R)v);
-            f.x = fma(-cp, xp.y, f.x);
-            f.y = fma(cp, xp.x, f.y);
-            f = cfma(t.minus_c, mk2<R>(sm * xm.x, sm * xm.y), f);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            if (!live[u])
+                continue;
+            C f = sig[lo[u]];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                const TermT<R> t = terms[e2[u] * NT + k];
+                const int v = nbv[u][k].x & 63;
+                const R cp = t.plus_c * sqrt((R)(v + 1));
+                const R sm = sqrt((R)v);
+                f.x = fma(-cp, xp[u][k].y, f.x);
+                f.y = fma(cp, xp[u][k].x, f.y);
+                f = cfma(t.minus_c, mk2<R>(sm * xm[u][k].x, sm * xm[u][k].y), f);
+            }
+            C r0, r1 = mk2<R>(R(0), R(0));
+            stage_values<STAGE, R>(a.c1, own[u], f, ep[u], r0, r1);
+            stage_store<STAGE, R>(a, o[u], r0, r1);
         }
-        stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, se, f);
+    }
+}
+
+template <typename R, int B, int NT, bool REALH>
+void launch_wb(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
+{
+    const dim3 grid((unsigned)(a.mp / T)), block(64 * T);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 1: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 2: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 3: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    default: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    }
+}
+
+template <typename R, int B, bool REALH>
+void launch_wn(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
+{
+    switch (nt) {
+    case 2: launch_wb<R, B, 2, REALH>(N, stage, a, el, T, st); break;
+    case 4: launch_wb<R, B, 4, REALH>(N, stage, a, el, T, st); break;
+    default: launch_wb<R, B, 8, REALH>(N, stage, a, el, T, st); break;
     }
 }
 
@@ -101,14 +216,13 @@ void launch_w(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArg
 {
     const int fit = wide_tile(N, (int)sizeof(c2<R>));
     const int T = ados > 0 && ados <= fit && kTile % ados == 0 ? ados : fit;
-    const dim3 grid((unsigned)(a.mp / T)), block(64 * T);
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage_wide<R, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
-    case 1: hipLaunchKernelGGL((rk4_stage_wide<R, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
-    case 2: hipLaunchKernelGGL((rk4_stage_wide<R, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
-    case 3: hipLaunchKernelGGL((rk4_stage_wide<R, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
-    default: hipLaunchKernelGGL((rk4_stage_wide<R, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, nt, T); break;
-    }
+    const int B = (N + 7) / 8;
+    if (B <= 2)
+        launch_wn<R, 2, REALH>(N, nt, stage, a, el, T, st);
+    else if (B == 3)
+        launch_wn<R, 3, REALH>(N, nt, stage, a, el, T, st);
+    else
+        launch_wn<R, 4, REALH>(N, nt, stage, a, el, T, st);
 }
 
 }  // namespace
