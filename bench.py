@@ -14,6 +14,7 @@ Prints one JSON line with ADO-RK4-steps/s, the roofline of the stage kernel and 
 CPU oracle timed on the host cores beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--no-cpu-baseline]
+                  [--no-extras | --extras precision,ensemble,adjoint]
 """
 from __future__ import annotations
 
@@ -41,6 +42,8 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the FP32-storage run of the same workload and the batched ensemble")
+    p.add_argument("--extras", default="precision,ensemble,adjoint",
+                   help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, adjoint")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--copy-mib", type=int, default=0,
@@ -345,11 +348,14 @@ def main():
         "tuning": {"kept": next(r for r in tuning if r["chosen"]), "candidates": len(tuning)},
     }
     h.close()
-    if not args.no_extras:
+    extras = set() if args.no_extras else {x.strip() for x in args.extras.split(",") if x.strip()}
+    if "precision" in extras:
         out["precision"] = precision_run(host, int(round(1e-12 / dt)))
         out["precision"]["sites24"] = wide_precision_run()
+    if "ensemble" in extras:
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value
+    if "adjoint" in extras:
         out["adjoint"] = adjoint_run(not args.no_cpu_baseline)
     host.close()
     if not args.no_cpu_baseline:
