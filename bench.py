@@ -52,6 +52,41 @@ def parse_args():
     return p.parse_args()
 
 
+def copy_ceiling(gib: float = 1.0, reps: int = 5) -> float:
+    """A plain device copy of `gib` GiB (read + write), the best of `reps`, in GB/s moved:
+    the HBM rate a streaming kernel can reach on this GPU."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / best / 1e9
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
 def cpu_baseline(cfg_path: str, target_s: float) -> dict:
     """The FP64 CPU oracle (statement-by-statement restatement of the reference kernels,
     OpenMP over ADOs) on a bounded sample of the same workload."""
@@ -80,6 +115,7 @@ def cpu_baseline(cfg_path: str, target_s: float) -> dict:
         "kind": "port",
         "sample": f"{n} RK4 steps of the same {inst.matrices}-ADO hierarchy after 1 warm-up step "
                   f"(unfused: 4 RHS + 4 weighted adds per step), {dt:.1f} s",
+        "host": host_cpu(),
     }
 
 
@@ -299,6 +335,7 @@ def main():
         del src, dst
     top = h.get_top()
     trace_err = abs(complex(top.trace()) - 1.0)
+    copy_gbs = copy_ceiling()
 
     value = M * args.steps / wall
     bytes_per_ado_step = 16 * N * N * 16  # 16 state passes of N^2 complex128 (algorithmic)
@@ -343,6 +380,8 @@ def main():
             "kernel": "rk4_stage (RHS + RK4 stage epilogue), 4 launches per step",
             "alg_bytes_per_launch": alg_bytes_per_launch,
             "avg_launch_ms": per_launch_s * 1e3,
+            "measured_copy_GBs": copy_gbs,
+            "frac_of_measured_copy": achieved / copy_gbs,
         },
         "checks": {"top_trace_minus_1": trace_err},
         "tuning": {"kept": next(r for r in tuning if r["chosen"]), "candidates": len(tuning)},
