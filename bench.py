@@ -42,8 +42,9 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the FP32-storage run of the same workload and the batched ensemble")
-    p.add_argument("--extras", default="precision,ensemble,adjoint",
-                   help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, adjoint")
+    p.add_argument("--extras", default="precision,ensemble,adjoint,filtering",
+                   help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, "
+                        "adjoint, filtering (compaction)")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--copy-mib", type=int, default=0,
@@ -265,6 +266,49 @@ def adjoint_run(cpu: bool) -> dict:
     return out
 
 
+def filtering_run(host, quantile: float = 0.5, settle_steps: int = 20, steps: int = 20) -> dict:
+    """Filtering with compaction on the same hierarchy: after `settle_steps` RK4 steps,
+    the ADOs whose squared norm lies below the `quantile` of all norms are masked
+    (heom_hierarchy_norm with a threshold) and removed from the device layout
+    (heom_compact).  Reports the step time before and after, and the deviation of the
+    compacted run's tops from the masked, uncompacted run."""
+    info = host.info
+    dt = info.step_size
+    handles = [host.create_handle(), host.create_handle()]
+    for h in handles:
+        h.step_rk4(dt, settle_steps)
+    norms = handles[0].norms()
+    threshold = float(np.quantile(norms, quantile))
+    for h in handles:
+        h.norms(threshold)
+    t0 = handles[1].timing().rk4_seconds
+    handles[1].step_rk4(dt, steps)
+    handles[1].sync()
+    before = (handles[1].timing().rk4_seconds - t0) / steps
+    kept = handles[0].compact()
+    handles[0].autotune(steps=2)
+    t0 = handles[0].timing().rk4_seconds
+    handles[0].step_rk4(dt, steps)
+    handles[0].sync()
+    after = (handles[0].timing().rk4_seconds - t0) / steps
+    # tops of both over the same further steps
+    a = handles[0].propagate(dt, steps, steps)
+    b = handles[1].propagate(dt, steps, steps)
+    dev = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    for h in handles:
+        h.close()
+    return {
+        "workload": f"{host.matrices}-ADO hierarchy after {settle_steps} steps; ADOs with squared norm below the "
+                    f"{quantile:.0%} quantile masked and compacted",
+        "threshold": threshold,
+        "ados_kept": kept,
+        "ms_per_step_masked": before * 1e3,
+        "ms_per_step_compacted": after * 1e3,
+        "speedup": before / after,
+        "max_rel_top_deviation_compacted_vs_masked": dev,
+    }
+
+
 def ensemble_run(members: int) -> dict:
     """The 512-realisation static-disorder ensemble (7 sites, depths 3-6, populations
     plus xx/yy/zz linear absorption, 2048 steps of 2 fs), batched, FP64 and FP32."""
@@ -391,6 +435,8 @@ def main():
     if "precision" in extras:
         out["precision"] = precision_run(host, int(round(1e-12 / dt)))
         out["precision"]["sites24"] = wide_precision_run()
+    if "filtering" in extras:
+        out["filtering"] = filtering_run(host)
     if "ensemble" in extras:
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value

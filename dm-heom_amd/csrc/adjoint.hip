@@ -294,13 +294,17 @@ void check_adjoint_supported(heom_handle* h)
     if (h->nel > 64)
         raise(HEOM_E_UNSUPPORTED, "the adjoint needs N*N <= 64");
     // the transpose of a filtered hierarchy masks destinations instead of sources;
-    // the adjoint kernels implement the unfiltered operator
+    // the adjoint kernels implement the unfiltered operator, so every ADO that is in the
+    // layout must be unmasked (ADOs removed by heom_compact are not in it)
     std::vector<uint8_t> mask(h->M);
+    std::vector<int32_t> q2u(h->Mp);
     HIPC(hipMemcpyAsync(mask.data(), h->mask.p, h->M, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipMemcpyAsync(q2u.data(), h->q2u.p, q2u.size() * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
-    for (int64_t u = 0; u < h->M; ++u)
-        if (mask[u] == 0)
-            raise(HEOM_E_UNSUPPORTED, "the adjoint needs an unfiltered hierarchy (no masked ADOs)");
+    for (int64_t q = 0; q < h->Mp; ++q)
+        if (q2u[q] >= 0 && mask[q2u[q]] == 0)
+            raise(HEOM_E_UNSUPPORTED,
+                  "the adjoint needs an unfiltered hierarchy (mask ADOs only together with heom_compact)");
 }
 
 // element constants of L^dag: -H^dag / hbar, conj(gam), conj(pf_same_sum), and per term

@@ -110,7 +110,9 @@ struct heom_handle {
     std::vector<int64_t> inst_M;    // ADOs per instance
     std::vector<int64_t> inst_u;    // first global uid of each instance
     std::vector<int64_t> inst_q;    // first internal index (a multiple of 64)
+    std::vector<int64_t> inst_n;    // internal slots holding ADOs (inst_M, or fewer after heom_compact)
     int64_t M = 0, Mp = 0;          // total ADOs (uid space), internal slots
+    int64_t active = 0;             // ADOs with a slot (M, or fewer after heom_compact)
     int ntiles = 0;
     int32_t zero_slot = 0;
     bool real_h = false;

@@ -199,6 +199,19 @@ __global__ void fold_mask(const int2* __restrict__ raw, int2* __restrict__ folde
     folded[t] = make_int2((int32_t)(((uint32_t)pf << 6) | (uint32_t)n), mf);
 }
 
+// new planes from old ones: dst[e * mp_new + q'] = src[e * mp_old + n2o[q']] (n2o < 0: zero)
+__global__ void remap_planes(const double2* __restrict__ src, double2* __restrict__ dst,
+                             const int32_t* __restrict__ n2o, int64_t mp_old, int64_t mp_new, int nel)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nel * mp_new)
+        return;
+    const int e = (int)(t / mp_new);
+    const int64_t qn = t % mp_new;
+    const int32_t qo = n2o[qn];
+    dst[t] = qo >= 0 ? src[(int64_t)e * mp_old + qo] : make_double2(0.0, 0.0);
+}
+
 // sigma_q <- Mx_k sigma_q for every ADO q of instance k: wave = row i, lane = ADO,
 // tile in LDS, arithmetic in FP64
 template <typename C, int N>
@@ -864,17 +877,20 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     h->inst_M.resize(count);
     h->inst_u.resize(count);
     h->inst_q.resize(count);
+    h->inst_n.resize(count);
     int64_t u0 = 0, q0 = 0;
     for (int k = 0; k < count; ++k) {
         h->inst_M[k] = d[k].matrices;
         h->inst_u[k] = u0;
         h->inst_q[k] = q0;
+        h->inst_n[k] = d[k].matrices;
         u0 += d[k].matrices;
         q0 += k + 1 < count ? (d[k].matrices + kTile - 1) / kTile * kTile : d[k].matrices;
     }
     if (q0 + 1 > kMaxAdos - kTile)
         raise(HEOM_E_UNSUPPORTED, "batch larger than 2^26 - 129 ADO slots");
     h->M = u0;
+    h->active = u0;
     h->zero_slot = (int32_t)q0;
     h->Mp = (q0 + 1 + kTile - 1) / kTile * kTile;
     h->ntiles = (int)(h->Mp / kTile);
@@ -991,6 +1007,8 @@ void download_state(heom_handle* h, const unsigned char* src, const double2* row
 {
     DevBuf<double2> stage;
     stage.alloc((size_t)h->M * h->nel);
+    if (h->active < h->M)  // ADOs removed by heom_compact read back as zero
+        HIPC(hipMemsetAsync(stage.p, 0, stage.bytes(), h->stream));
     if (h->fp32())
         hipLaunchKernelGGL(soa_to_aos<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
                            reinterpret_cast<const float2*>(src), stage.p, h->q2u.p, h->Mp, h->nel);
@@ -1219,6 +1237,8 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
         need(h);
         if (!norms)
             raise(HEOM_E_INVALID, "norms is NULL");
+        if (h->active < h->M)  // ADOs removed by heom_compact have norm 0
+            HIPC(hipMemsetAsync(h->norms.p, 0, h->norms.bytes(), h->stream));
         if (h->fp32())
             hipLaunchKernelGGL(hierarchy_norm_kernel<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
                                h->y.as<float2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p, h->mask.p);
@@ -1427,7 +1447,7 @@ int heom_rk4_step_bytes(heom_handle* h, int64_t* bytes)
         need(h);
         if (!bytes)
             raise(HEOM_E_INVALID, "bytes is NULL");
-        *bytes = (int64_t)16 * h->M * h->nel * (int64_t)h->cbytes();
+        *bytes = (int64_t)16 * h->active * h->nel * (int64_t)h->cbytes();
     });
 }
 
@@ -1438,6 +1458,113 @@ int heom_stage_kernel(heom_handle* h, int* kind)
         if (!kind)
             raise(HEOM_E_INVALID, "kind is NULL");
         *kind = h->N > kMaxStates ? HEOM_STAGE_WIDE : h->pipelined ? HEOM_STAGE_PIPELINED : HEOM_STAGE_TILE;
+    });
+}
+
+// Hierarchy compaction: the device layout is rebuilt with only the ADOs whose mask is
+// not FALSE.  A masked ADO is never read by its neighbours (heom_ode.cl:199,237,297,360),
+// so the unmasked ADOs evolve exactly as before and every observable of them is
+// unchanged; the masked ones no longer cost bytes.  Each instance keeps its ADOs in the
+// same relative order, padded to 64 slots; edges to removed ADOs read the zero slot.
+int heom_compact(heom_handle* h, int64_t* active)
+{
+    return guard([&] {
+        need(h);
+        if (h->fp32())
+            raise(HEOM_E_UNSUPPORTED, "compaction needs FP64 storage");
+        const int W = h->W, nel = h->nel;
+        const int64_t Mp = h->Mp;
+        std::vector<uint8_t> mask(h->M);
+        std::vector<int32_t> q2u(Mp);
+        std::vector<int2> nb((size_t)(W + 2) * Mp);
+        std::vector<double2> pfss(Mp);
+        HIPC(hipMemcpyAsync(mask.data(), h->mask.p, h->M, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(q2u.data(), h->q2u.p, q2u.size() * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(nb.data(), h->nb_raw.p, nb.size() * sizeof(int2), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(pfss.data(), h->pfss.p, pfss.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                            h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        std::vector<int64_t> qtop(h->count);
+        HIPC(hipMemcpy(qtop.data(), h->q_top.p, qtop.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+
+        // new slots, instance by instance
+        std::vector<int32_t> o2n(Mp, -1);
+        std::vector<int64_t> new_q(h->count), new_n(h->count);
+        int64_t q0 = 0, kept = 0;
+        for (int k = 0; k < h->count; ++k) {
+            new_q[k] = q0;
+            const int64_t lo = h->inst_q[k], hi = h->inst_q[k] + h->inst_n[k];
+            int64_t n = 0;
+            for (int64_t q = lo; q < hi; ++q)
+                if (q2u[q] >= 0 && mask[q2u[q]] != 0)
+                    o2n[q] = (int32_t)(q0 + n++);
+            if (o2n[qtop[k]] < 0)
+                raise(HEOM_E_INVALID, "the top ADO of every instance must stay unmasked");
+            kept += n;
+            new_n[k] = n;
+            q0 += k + 1 < h->count ? (n + kTile - 1) / kTile * kTile : n;
+        }
+        const int32_t zero = (int32_t)q0;
+        const int64_t mp = (q0 + 1 + kTile - 1) / kTile * kTile;
+        const int ntiles = (int)(mp / kTile);
+        std::vector<int32_t> n2o(mp, -1), q2u_n(mp, -1), tinst(ntiles, h->count - 1);
+        std::vector<double2> pfss_n(mp, make_double2(0.0, 0.0));
+        std::vector<int2> nb_n((size_t)(W + 2) * mp, make_int2((int32_t)((uint32_t)zero << 6), zero));
+        for (int64_t q = 0; q < Mp; ++q) {
+            const int32_t qn = o2n[q];
+            if (qn < 0)
+                continue;
+            n2o[qn] = (int32_t)q;
+            q2u_n[qn] = q2u[q];
+            pfss_n[qn] = pfss[q];
+            for (int m = 0; m < W; ++m) {
+                const int2 r = nb[(size_t)m * Mp + q];
+                const int32_t p = (int32_t)((uint32_t)r.x >> 6), nm = r.x & 63;
+                const int32_t pn = p == h->zero_slot ? zero : (o2n[p] >= 0 ? o2n[p] : zero);
+                const int32_t mn = r.y == h->zero_slot ? zero : (o2n[r.y] >= 0 ? o2n[r.y] : zero);
+                nb_n[(size_t)m * mp + qn] = make_int2((int32_t)(((uint32_t)pn << 6) | (uint32_t)nm), mn);
+            }
+        }
+        for (int k = 0; k < h->count; ++k) {
+            for (int64_t t = new_q[k] / kTile; t < (new_q[k] + new_n[k] + kTile - 1) / kTile; ++t)
+                tinst[t] = k;
+            qtop[k] = o2n[qtop[k]];
+        }
+
+        // move the state and swap in the new layout
+        DevBuf<int32_t> n2o_d;
+        upload(n2o_d, n2o, h->stream);
+        DevBuf<unsigned char> y_new;
+        y_new.alloc((size_t)nel * mp * sizeof(double2));
+        hipLaunchKernelGGL(remap_planes, dim3(blocks_for((int64_t)nel * mp, 256)), dim3(256), 0, h->stream,
+                           h->y.as<double2>(), y_new.as<double2>(), n2o_d.p, Mp, mp, nel);
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(h->stream));
+        std::swap(h->y.p, y_new.p);
+        std::swap(h->y.n, y_new.n);
+        for (auto* b : {&h->pbuf, &h->ya, &h->yb}) {
+            b->alloc((size_t)nel * mp * sizeof(double2));
+            HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+        }
+        upload(h->nb_raw, nb_n, h->stream);
+        upload(h->nb, nb_n, h->stream);
+        upload(h->pfss, pfss_n, h->stream);
+        upload(h->q2u, q2u_n, h->stream);
+        upload(h->tile_inst, tinst, h->stream);
+        upload(h->q_top, qtop, h->stream);
+        h->Mp = mp;
+        h->ntiles = ntiles;
+        h->zero_slot = zero;
+        h->active = kept;
+        h->inst_q = new_q;
+        h->inst_n = new_n;
+        refold_tables(h);
+        drop_graph(h);
+        adjoint_invalidate(h);
+        choose_stage_kernel(h);
+        HIPC(hipStreamSynchronize(h->stream));
+        if (active)
+            *active = kept;
     });
 }
 

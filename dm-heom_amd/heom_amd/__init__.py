@@ -38,7 +38,7 @@ EXPORTED = [
     "heom_get_state", "heom_get_top", "heom_hierarchy_norm", "heom_rhs", "heom_step_rk4",
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
-    "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune",
+    "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -130,6 +130,7 @@ def load():
             "heom_rhs_adjoint": [vp, vp],
             "heom_autotune": [vp, ctypes.c_int32, ctypes.POINTER(TuneResult), ctypes.c_int32,
                               ctypes.POINTER(ctypes.c_int32)],
+            "heom_compact": [vp, ctypes.POINTER(i64)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -384,6 +385,12 @@ class Handle:
         k = ctypes.c_int()
         _check(load().heom_stage_kernel(self._h, ctypes.byref(k)))
         return STAGE_KINDS[k.value]
+
+    def compact(self) -> int:
+        """Drop the masked ADOs from the device layout; returns the number kept."""
+        n = ctypes.c_int64()
+        _check(load().heom_compact(self._h, ctypes.byref(n)))
+        return n.value
 
     def autotune(self, steps: int = 3) -> list:
         """Time every stage-kernel configuration on this handle's state (restored

@@ -188,6 +188,16 @@ int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
 enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1, HEOM_STAGE_WIDE = 2 };
 int heom_stage_kernel(heom_handle* handle, int* kind);
 
+/* Hierarchy compaction (filtering with compaction; no counterpart in the reference,
+ * which keeps computing masked ADOs): rebuilds the device layout with only the ADOs
+ * whose mask is not FALSE (the mask of heom_set_mask, or of heom_hierarchy_norm with a
+ * threshold).  A masked ADO is never read by its neighbours, so the other ADOs -- the
+ * tops, populations and traces among them -- evolve exactly as before, at the cost of
+ * the remaining ADOs only.  Removed ADOs read back as zero and stay removed (a later mask
+ * cannot bring them back); the top ADO of every instance must be unmasked.  FP64
+ * handles.  active receives the number of ADOs kept (may be NULL). */
+int heom_compact(heom_handle* handle, int64_t* active);
+
 /* The autotuner (no counterpart in the reference): times every valid configuration of
  * the stage kernels for this handle -- tile or pipelined kernel and tiles per XCD chunk
  * of the tile order (N <= 8), ADOs per workgroup (N > 8, HEOM_STAGE_WIDE) -- over `steps`
