@@ -42,9 +42,9 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the FP32-storage run of the same workload and the batched ensemble")
-    p.add_argument("--extras", default="precision,ensemble,adjoint,filtering",
+    p.add_argument("--extras", default="precision,ensemble,adjoint,filtering,field",
                    help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, "
-                        "adjoint, filtering (compaction)")
+                        "adjoint, filtering (compaction), field (time-dependent H)")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--copy-mib", type=int, default=0,
@@ -309,6 +309,35 @@ def filtering_run(host, quantile: float = 0.5, settle_steps: int = 20, steps: in
     }
 
 
+def field_run(host, steps: int = 40) -> dict:
+    """A time-dependent Hamiltonian (pump/probe field, heom_propagate_timedep) on the same
+    hierarchy: step time against the constant-H step time."""
+    import heom_amd
+    from heom_amd.field import pump_probe, stage_hamiltonians
+
+    info = host.info
+    dt = info.step_size
+    N = host.states
+    rng = np.random.default_rng(7)
+    dm = rng.normal(size=(N, N)) * 1e-21
+    H = host.arrays()["hamiltonian"].reshape(N, N, 2)
+    H0 = H[..., 0] + 1j * H[..., 1]
+    field = pump_probe(probe=dict(efield=2.0, center=20e-15, width=10e-15, phase=0.3, frequency=2.3e15),
+                       pump=dict(efield=5.0, center=10e-15, width=6e-15, phase=-0.5, frequency=2.25e15))
+    sh = stage_hamiltonians(H0, dm, dm.T, field, 0.0, dt, steps)
+    h = host.create_handle()
+    h.propagate_timedep(dt, 2, 2, sh[:2])
+    t0 = h.timing().rk4_seconds
+    h.propagate_timedep(dt, steps, steps, sh)
+    t_field = (h.timing().rk4_seconds - t0) / steps
+    t0 = h.timing().rk4_seconds
+    h.propagate(dt, steps, steps)
+    t_plain = (h.timing().rk4_seconds - t0) / steps
+    h.close()
+    return {"workload": f"pump/probe field on the {host.matrices}-ADO hierarchy, {steps} RK4 steps",
+            "ms_per_step_field": t_field * 1e3, "ms_per_step_constant_h": t_plain * 1e3}
+
+
 def ensemble_run(members: int) -> dict:
     """The 512-realisation static-disorder ensemble (7 sites, depths 3-6, populations
     plus xx/yy/zz linear absorption, 2048 steps of 2 fs), batched, FP64 and FP32."""
@@ -437,6 +466,8 @@ def main():
         out["precision"]["sites24"] = wide_precision_run()
     if "filtering" in extras:
         out["filtering"] = filtering_run(host)
+    if "field" in extras:
+        out["field"] = field_run(host)
     if "ensemble" in extras:
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value

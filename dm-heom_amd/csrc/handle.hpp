@@ -144,6 +144,11 @@ struct heom_handle {
     DevBuf<int2> sp_nb;
     DevBuf<double2> y64, sa64, sb64, p64, sp_pfss, sp_tmp;
 
+    // time-dependent Hamiltonian (heom_propagate_timedep): H / hbar of the current stage,
+    // [count][nel], in place of hs / hs32 while set
+    const double2* hs_stage = nullptr;
+    const float2* hs32_stage = nullptr;
+
     // one RK4 step captured as a HIP graph (heom_use_graphs), replayed per step
     bool use_graphs = false;
     hipGraphExec_t step_graph = nullptr;

@@ -476,7 +476,7 @@ void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsi
     a.out64 = stage == 2 ? h->sb64.p : stage == 4 ? h->y64.p : h->sa64.p;
     a.y64 = h->y64.p;
     a.p64 = h->p64.p;
-    a.hs = h->hs.p;
+    a.hs = h->hs_stage ? h->hs_stage : h->hs.p;
     a.gam = h->gam.p;
     a.terms = h->terms.p;
     a.pfss = h->sp_pfss.p;
@@ -502,11 +502,11 @@ void launch_stage(heom_handle* h, int stage, const unsigned char* yin, unsigned 
                   bool reverse = false)
 {
     if (h->fp32()) {
-        const ElementArgsT<float> el{h->hs32.p, h->gam32.p, h->terms32.p};
+        const ElementArgsT<float> el{h->hs32_stage ? h->hs32_stage : h->hs32.p, h->gam32.p, h->terms32.p};
         launch_stage_r<float>(h, stage, yin, yout, c1, reverse, el, h->pfss32.p);
         tier_fixup_launch(h, stage, yin, yout, c1);
     } else {
-        const ElementArgsT<double> el{h->hs.p, h->gam.p, h->terms.p};
+        const ElementArgsT<double> el{h->hs_stage ? h->hs_stage : h->hs.p, h->gam.p, h->terms.p};
         launch_stage_r<double>(h, stage, yin, yout, c1, reverse, el, h->pfss.p);
     }
     h->timing.rhs_launches += 1;
@@ -1316,6 +1316,88 @@ int heom_propagate(heom_handle* h, double dt, int64_t nsteps, int64_t observe_st
         h->pending_steps = nsteps;
         HIPC(hipMemcpyAsync(tops, rec.p, rec.bytes(), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
+        resolve_timing(h);
+    });
+}
+
+// The reference's e_field pre-evaluate action (dm-heom/include/heom/e_field.hpp:64-87)
+// sets H before every RHS evaluation to its value at that evaluation's time; the caller
+// passes those Hamiltonians for the RK4 stage times t, t + h/2, t + h/2, t + h.  They are
+// uploaded in chunks of steps as H / hbar, and each stage launch reads its own.
+int heom_propagate_timedep(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps,
+                           const double* stage_h, double* tops)
+{
+    return guard([&] {
+        need(h);
+        if (!tops || !stage_h)
+            raise(HEOM_E_INVALID, "tops and stage_hamiltonians must not be NULL");
+        if (nsteps < 0 || observe_steps < 1)
+            raise(HEOM_E_INVALID, "nsteps must be >= 0 and observe_steps >= 1");
+        if (!(dt > 0.0))
+            raise(HEOM_E_INVALID, "step size must be > 0");
+        const int64_t rows = nsteps / observe_steps + 1;
+        const int64_t per = (int64_t)h->count * h->nel;  // one stage's Hamiltonians
+        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsteps, (64 << 20) / (4 * per * 16)));
+        DevBuf<double2> rec, hsd;
+        DevBuf<float2> hsd32;
+        rec.alloc((size_t)(rows * per));
+        hsd.alloc((size_t)(chunk * 4 * per));
+        if (h->fp32())
+            hsd32.alloc((size_t)(chunk * 4 * per));
+        std::vector<double2> hb((size_t)(chunk * 4 * per));
+        std::vector<float2> hb32(h->fp32() ? hb.size() : 0);
+        bool real = true;
+        for (int64_t t = 0; t < nsteps * 4 * per; ++t)
+            if (stage_h[2 * t + 1] != 0.0)
+                real = false;
+        const bool real_kept = h->real_h;
+        h->real_h = real;  // the kernels read the stage Hamiltonians only
+        resolve_timing(h);
+        try {
+            gather_tops(h, rec.p);
+            HIPC(hipEventRecord(h->ev_start, h->stream));
+            int64_t row = 1;
+            for (int64_t c0 = 0; c0 < nsteps; c0 += chunk) {
+                const int64_t nc = std::min(chunk, nsteps - c0);
+                HIPC(hipStreamSynchronize(h->stream));  // the previous chunk's Hamiltonians are consumed
+                for (int64_t t = 0; t < nc * 4 * per; ++t) {
+                    const double* src = stage_h + 2 * (c0 * 4 * per + t);
+                    hb[t] = make_double2(src[0] / kHbar, src[1] / kHbar);
+                    if (h->fp32())
+                        hb32[t] = make_float2((float)hb[t].x, (float)hb[t].y);
+                }
+                HIPC(hipMemcpyAsync(hsd.p, hb.data(), (size_t)(nc * 4 * per) * sizeof(double2),
+                                    hipMemcpyHostToDevice, h->stream));
+                if (h->fp32())
+                    HIPC(hipMemcpyAsync(hsd32.p, hb32.data(), (size_t)(nc * 4 * per) * sizeof(float2),
+                                        hipMemcpyHostToDevice, h->stream));
+                for (int64_t n = 0; n < nc; ++n) {
+                    const double c1[4] = {0.5 * dt, 0.5 * dt, dt, dt / 6.0};
+                    unsigned char* in[4] = {h->y.p, h->ya.p, h->yb.p, h->ya.p};
+                    unsigned char* out[4] = {h->ya.p, h->yb.p, h->ya.p, h->y.p};
+                    for (int s = 0; s < 4; ++s) {
+                        h->hs_stage = hsd.p + (n * 4 + s) * per;
+                        h->hs32_stage = h->fp32() ? hsd32.p + (n * 4 + s) * per : nullptr;
+                        launch_stage(h, s + 1, in[s], out[s], c1[s], (s & 1) != 0);
+                    }
+                    h->hs_stage = nullptr;
+                    h->hs32_stage = nullptr;
+                    if ((c0 + n + 1) % observe_steps == 0)
+                        gather_tops(h, rec.p + (row++) * per);
+                }
+            }
+            HIPC(hipEventRecord(h->ev_stop, h->stream));
+            h->timing_pending = true;
+            h->pending_steps = nsteps;
+            HIPC(hipMemcpyAsync(tops, rec.p, rec.bytes(), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+        } catch (...) {
+            h->hs_stage = nullptr;
+            h->hs32_stage = nullptr;
+            h->real_h = real_kept;
+            throw;
+        }
+        h->real_h = real_kept;
         resolve_timing(h);
     });
 }

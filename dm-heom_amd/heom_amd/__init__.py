@@ -39,6 +39,7 @@ EXPORTED = [
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
+    "heom_propagate_timedep",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -131,6 +132,7 @@ def load():
             "heom_autotune": [vp, ctypes.c_int32, ctypes.POINTER(TuneResult), ctypes.c_int32,
                               ctypes.POINTER(ctypes.c_int32)],
             "heom_compact": [vp, ctypes.POINTER(i64)],
+            "heom_propagate_timedep": [vp, dbl, i64, i64, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -342,6 +344,25 @@ class Handle:
         t = t[..., 0] + 1j * t[..., 1]
         return t[:, 0] if self.count == 1 else t
 
+    def propagate_timedep(self, h: float, nsteps: int, observe_steps: int, stage_h) -> np.ndarray:
+        """propagate() with the Hamiltonian of every RK4 stage given: stage_h complex
+        [nsteps, 4, N, N] (or [nsteps, 4, count, N, N]) in J, at the stage times t_n,
+        t_n + h/2, t_n + h/2, t_n + h (heom_amd.field.stage_hamiltonians builds them)."""
+        N = self.states
+        sh = np.asarray(stage_h, dtype=np.complex128)
+        if sh.ndim == 4:
+            sh = np.broadcast_to(sh[:, :, None], (nsteps, 4, self.count, N, N))
+        if sh.shape != (nsteps, 4, self.count, N, N):
+            raise ValueError("stage_h must be [nsteps, 4, (count,) N, N]")
+        flat = _c128_flat(sh)
+        rows = nsteps // observe_steps + 1
+        out = np.empty(rows * self.count * N * N * 2)
+        _check(load().heom_propagate_timedep(self._h, float(h), int(nsteps), int(observe_steps), flat.ctypes.data,
+                                             out.ctypes.data))
+        t = out.reshape(rows, self.count, N, N, 2)
+        t = t[..., 0] + 1j * t[..., 1]
+        return t[:, 0] if self.count == 1 else t
+
     def propagate_observe(self, h: float, nsteps: int, observe_steps: int, ops=None):
         """Populations (rows, count, N) and, with ops (an N x N operator, or one per
         instance), traces Tr(ops sigma_0) (rows, count), reduced on the device."""
@@ -454,6 +475,7 @@ HOST_EXPORTED = [
     "heom_host_open", "heom_host_close", "heom_host_last_error", "heom_host_get_info", "heom_host_desc",
     "heom_host_top", "heom_host_cbk_sqrt", "heom_host_mask", "heom_host_dipoles", "heom_host_partitions",
     "heom_host_set_parameters", "heom_host_get_parameters", "heom_host_parameter_gradient",
+    "heom_host_write_instance", "heom_host_read_instance_state",
 ]
 
 
@@ -493,6 +515,8 @@ def load_host():
             "heom_host_set_parameters": ([vp, vp, vp, vp, vp], ctypes.c_int),
             "heom_host_get_parameters": ([vp, vp, vp, vp, vp], ctypes.c_int),
             "heom_host_parameter_gradient": ([vp, ctypes.POINTER(Gradients), vp, vp, vp, vp], ctypes.c_int),
+            "heom_host_write_instance": ([vp, ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+            "heom_host_read_instance_state": ([vp, ctypes.c_char_p, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -641,6 +665,24 @@ class HostInstance:
                                                              dl.ctypes.data, dg.ctypes.data, do.ctypes.data))
         dH = dH.reshape(self.sites, self.sites, 2)
         return {"hamiltonian": dH[..., 0] + 1j * dH[..., 1], "lambda": dl, "gamma": dg, "omega": do}
+
+    def write_instance(self, path: str, state=None, top_only: bool = False, time: float = 0.0) -> None:
+        """Write the reference's binary instance file with `state` (reference layout), or
+        the top ADO only."""
+        s = _f64(state) if state is not None else None
+        if s is not None and s.size != self.matrices * self.states * self.states * 2:
+            raise ValueError("state has the wrong size")
+        _host_check(load_host().heom_host_write_instance(self._p, str(path).encode(),
+                                                         s.ctypes.data if s is not None else None,
+                                                         int(bool(top_only) or s is None), float(time)))
+
+    def read_instance_state(self, path: str):
+        """(state, time) from a binary instance file of this instance."""
+        st = np.zeros(self.matrices * self.states * self.states * 2)
+        t = ctypes.c_double()
+        _host_check(load_host().heom_host_read_instance_state(self._p, str(path).encode(), st.ctypes.data,
+                                                              ctypes.byref(t)))
+        return st, t.value
 
     def create_handle(self, precision: int = HEOM_PRECISION_FP64) -> Handle:
         """heom_create on the instance, with its initial top ADO loaded."""

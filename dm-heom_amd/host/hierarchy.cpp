@@ -2,6 +2,8 @@
 #include "hierarchy.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <fstream>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
@@ -428,6 +430,117 @@ heom_desc instance::desc() const
     d.pf_circ = pf_circ_.data();
     d.cbk = cbk_.data();
     return d;
+}
+
+namespace {
+
+template <typename T>
+void put(std::ofstream& f, const T& v)
+{
+    f.write(reinterpret_cast<const char*>(&v), sizeof(T));
+}
+
+template <typename T>
+void put_array(std::ofstream& f, const T* p, std::size_t n)
+{
+    f.write(reinterpret_cast<const char*>(p), (std::streamsize)(n * sizeof(T)));
+}
+
+template <typename T>
+T get(std::ifstream& f)
+{
+    T v{};
+    f.read(reinterpret_cast<char*>(&v), sizeof(T));
+    if (!f)
+        throw config_error("instance file is truncated");
+    return v;
+}
+
+template <typename T>
+std::vector<T> get_array(std::ifstream& f, std::size_t n)
+{
+    std::vector<T> v(n);
+    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(T)));
+    if (!f)
+        throw config_error("instance file is truncated");
+    return v;
+}
+
+// enum values of the format: version_t::V_1, memory_layout_t::VANILLA,
+// sites_to_states_mode_t {identity, with_ground_state, excited_state_absorption}
+constexpr std::int32_t kVersion1 = 0, kVanilla = 0;
+
+}  // namespace
+
+void instance::write_file(const std::string& path, const double* state, bool top_only, double time) const
+{
+    std::ofstream f(path, std::ios::out | std::ios::binary | std::ios::trunc);
+    if (!f)
+        throw config_error("could not open instance file for writing: " + path);
+    const std::size_t nn2 = (std::size_t)(states_ * states_ * 2), M = (std::size_t)matrices_,
+                      W = (std::size_t)width_, B = (std::size_t)baths_, K = (std::size_t)matsubaras_;
+    put<std::int32_t>(f, kVersion1);
+    put<bool>(f, top_only);
+    put<std::int32_t>(f, kVanilla);
+    put<std::int32_t>(f, kVanilla);
+    put<std::int32_t>(f, (std::int32_t)mode_);  // same enumerator order as the format
+    put<double>(f, time);
+    put<std::int32_t>(f, (std::int32_t)first_non_plus_);
+    for (int_t v : {states_, matrices_, matsubaras_, baths_per_state_, baths_, width_, depth_})
+        put<std::int64_t>(f, (std::int64_t)v);
+    put_array(f, state, top_only ? nn2 : nn2 * M);
+    put_array(f, hamiltonian_.data(), nn2);
+    put_array(f, graph_.tuples().data(), M * W);
+    put_array(f, graph_.plus().data(), M * W);
+    put_array(f, graph_.minus().data(), M * W);
+    put_array(f, pf_same_sum_.data(), 2 * M);
+    put_array(f, pf_same_.data(), 2 * B);
+    put_array(f, pf_circ_.data(), 2 * B);
+    put_array(f, cbk_.data(), 2 * B * K);
+    put_array(f, cbk_sqrt_.data(), B * K);
+    if (!f)
+        throw config_error("could not write instance file: " + path);
+}
+
+void instance::read_state(const std::string& path, double* state, double* time) const
+{
+    std::ifstream f(path, std::ios::in | std::ios::binary);
+    if (!f)
+        throw config_error("could not open instance file: " + path);
+    if (get<std::int32_t>(f) != kVersion1)
+        throw config_error("instance file: unknown format version");
+    const bool top_only = get<bool>(f);
+    if (get<std::int32_t>(f) != kVanilla || get<std::int32_t>(f) != kVanilla)
+        throw config_error("instance file: unknown memory layout");
+    const std::int32_t mode = get<std::int32_t>(f);
+    const double t = get<double>(f);
+    const std::int32_t fnp = get<std::int32_t>(f);
+    std::int64_t sizes[7];
+    for (auto& v : sizes)
+        v = get<std::int64_t>(f);
+    const std::int64_t want[7] = {states_, matrices_, matsubaras_, baths_per_state_, baths_, width_, depth_};
+    for (int k = 0; k < 7; ++k)
+        if (sizes[k] != want[k])
+            throw config_error("instance file does not match this instance (sizes)");
+    if (mode != (std::int32_t)mode_ || fnp != (std::int32_t)first_non_plus_)
+        throw config_error("instance file does not match this instance (states mapping or graph)");
+    const std::size_t nn2 = (std::size_t)(states_ * states_ * 2), M = (std::size_t)matrices_,
+                      W = (std::size_t)width_;
+    const std::vector<double> hier = get_array<double>(f, top_only ? nn2 : nn2 * M);
+    (void)get_array<double>(f, nn2);  // the Hamiltonian at write time (it may have changed since)
+    if (get_array<std::int32_t>(f, M * W) != graph_.tuples() || get_array<std::int32_t>(f, M * W) != graph_.plus() ||
+        get_array<std::int32_t>(f, M * W) != graph_.minus())
+        throw config_error("instance file does not match this instance (ADO tables)");
+    // the prefactors of the file's parameters: read to check the length only, this
+    // instance's own (its current parameters) stay in use
+    const std::size_t B = (std::size_t)baths_, K = (std::size_t)matsubaras_;
+    (void)get_array<double>(f, 2 * M + 4 * B + 3 * B * K);
+    if (f.peek() != std::ifstream::traits_type::eof())
+        throw config_error("instance file has trailing data");
+    std::fill(state, state + nn2 * M, 0.0);
+    std::memcpy(state, hier.data(), hier.size() * sizeof(double));
+    if (time)
+        *time = t;
 }
 
 }  // namespace heom

@@ -114,6 +114,15 @@ public:
     // descriptor of the kernel inputs for heom_create (points into this instance)
     heom_desc desc() const;
 
+    // The reference's binary instance format (dm-heom/src/heom/instance.cpp:281-377,
+    // fields of dm-heom/include/heom/instance.hpp:182-211): header, then the hierarchy
+    // (all ADOs, or the top only) and every kernel-input array.  write_file stores this
+    // instance with `state` [M*N*N*2] (or its top ADO when top_only); read_state checks
+    // that a file describes this instance (sizes, graph tables) and returns its
+    // hierarchy (zeros below the top for a top-only file) and time.
+    void write_file(const std::string& path, const double* state, bool top_only, double time) const;
+    void read_state(const std::string& path, double* state, double* time) const;
+
 private:
     void precompute();
 

@@ -272,6 +272,28 @@ int heom_host_parameter_gradient(const heom_host_instance* p, const heom_gradien
     });
 }
 
+int heom_host_write_instance(const heom_host_instance* p, const char* path, const double* state, int top_only,
+                             double time)
+{
+    return host_guard([&] {
+        need(p);
+        if (!path)
+            throw std::invalid_argument("path is NULL");
+        const double* s = state ? state : p->inst->top().data();
+        p->inst->write_file(path, s, top_only != 0 || !state, time);
+    });
+}
+
+int heom_host_read_instance_state(const heom_host_instance* p, const char* path, double* state, double* time)
+{
+    return host_guard([&] {
+        need(p);
+        if (!path || !state)
+            throw std::invalid_argument("path and state must not be NULL");
+        p->inst->read_state(path, state, time);
+    });
+}
+
 int64_t heom_host_partitions(int n, int s, int32_t* out, int64_t capacity)
 {
     int64_t count = -1;

@@ -162,6 +162,14 @@ int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observ
  * FP32 storage, on the FP64 top), one wavefront per instance. */
 int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                            const double* ops, double* pops, double* traces);
+/* A time-dependent Hamiltonian: the reference's e_field pre-evaluate action
+ * (dm-heom/include/heom/e_field.hpp:64-87, hooked in at include/heom/ode.hpp:57-59) sets
+ * H = H0 - E(t) D- - conj(E(t)) D+ before every RHS evaluation, at that evaluation's
+ * time.  heom_propagate_timedep is heom_propagate with those Hamiltonians given for the
+ * RK4 stage times t_n, t_n + h/2, t_n + h/2, t_n + h of every step:
+ * stage_hamiltonians [nsteps][4][count][N*N*2] (J).  Steps are not graph-captured. */
+int heom_propagate_timedep(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
+                           const double* stage_hamiltonians, double* tops /* [rows][count][N*N*2] */);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
  * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */
 int heom_mmult_left(heom_handle* handle, const double* mx /* [count][N*N*2] */);

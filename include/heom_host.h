@@ -75,6 +75,15 @@ int heom_host_get_parameters(const heom_host_instance* inst, double* hamiltonian
  * d_gamma, d_omega [B].  Any output may be NULL. */
 int heom_host_parameter_gradient(const heom_host_instance* inst, const heom_gradients* grads,
                                  double* d_hamiltonian, double* d_lambda, double* d_gamma, double* d_omega);
+/* The reference's binary instance format (dm-heom/src/heom/instance.cpp:281-377): a
+ * checkpoint of a hierarchy with every kernel-input array.  write stores this instance
+ * with the state [M*N*N*2] (state NULL or top_only: the top ADO only) and a time;
+ * read_instance_state checks that the file describes this instance (sizes, ADO tables)
+ * and returns its hierarchy (zeros below the top for a top-only file) and time.  The
+ * format holds no bath coupling list, so a file is read back against its config. */
+int heom_host_write_instance(const heom_host_instance* inst, const char* path, const double* state, int top_only,
+                             double time);
+int heom_host_read_instance_state(const heom_host_instance* inst, const char* path, double* state, double* time);
 /* integer partitions of n into at most s parts in the reference order; writes
  * min(count, capacity) rows of s entries, returns count (negative on error) */
 int64_t heom_host_partitions(int n, int s, int32_t* out, int64_t capacity);

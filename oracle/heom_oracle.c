@@ -1219,3 +1219,56 @@ int oracle_adjoint_gradient(const oracle_instance* inst, double h, int64_t nstep
     free(wk);
     return 0;
 }
+
+/* ======================================================================== */
+/* time-dependent Hamiltonian (include/heom/e_field.hpp:64-87)              */
+/* ======================================================================== */
+/* The reference updates H before every RHS evaluation with the field at that
+ * evaluation's time (a pre-evaluate action of the ODE, include/heom/ode.hpp:57-59); for
+ * classical RK4 the stage times are t, t + h/2, t + h/2, t + h.  Here the four stage
+ * Hamiltonians of each step are given: H4[s * 2NN .. ] for stage s. */
+void oracle_rk4_step_timedep(const oracle_instance* inst, double h, double* y, double* work, const double* H4)
+{
+    const int64_t size = inst->matrices * inst->states * inst->states * 2;
+    const int64_t nn2 = (int64_t)inst->states * inst->states * 2;
+    oracle_instance st = *inst;
+    double* k1 = work;
+    double* k2 = work + size;
+    double* k3 = work + 2 * size;
+    double* k4 = work + 3 * size;
+    double* t = work + 4 * size;
+    const double* ks[4] = {k1, k2, k3, k4};
+    const double a2[1] = {0.5};
+    const double a3[2] = {0.0, 0.5};
+    const double a4[3] = {0.0, 0.0, 1.0};
+    const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+    st.hamiltonian = H4;
+    oracle_rhs(&st, y, k1);
+    oracle_rk_weighted_add(size, 1, h, t, y, a2, ks);
+    st.hamiltonian = H4 + nn2;
+    oracle_rhs(&st, t, k2);
+    oracle_rk_weighted_add(size, 2, h, t, y, a3, ks);
+    st.hamiltonian = H4 + 2 * nn2;
+    oracle_rhs(&st, t, k3);
+    oracle_rk_weighted_add(size, 3, h, t, y, a4, ks);
+    st.hamiltonian = H4 + 3 * nn2;
+    oracle_rhs(&st, t, k4);
+    oracle_rk_weighted_add(size, 4, h, t, y, b, ks);
+    memcpy(y, t, sizeof(double) * (size_t)size);
+}
+
+int64_t oracle_propagate_timedep(const oracle_instance* inst, double h, int64_t nsteps, int64_t observe_steps,
+                                 double* y, double* work, const double* stage_h, double* tops)
+{
+    const int64_t top = (int64_t)inst->states * inst->states * 2;
+    int64_t rows = 0;
+    if (tops)
+        memcpy(tops + (rows++) * top, y, sizeof(double) * (size_t)top);
+    for (int64_t s = 1; s <= nsteps; ++s) {
+        oracle_rk4_step_timedep(inst, h, y, work, stage_h + (s - 1) * 4 * top);
+        if (tops && observe_steps > 0 && s % observe_steps == 0)
+            memcpy(tops + (rows++) * top, y, sizeof(double) * (size_t)top);
+    }
+    return rows;
+}

@@ -94,6 +94,12 @@ void oracle_mmult_left(int states, int64_t matrices, const double* mx, double* s
 /* Tr(D * top) (include/heom/matrix_trace_observer.hpp:139-158); D NULL = identity */
 void oracle_trace(int states, const double* d, const double* top, double* trace2);
 
+/* ---- time-dependent Hamiltonian (include/heom/e_field.hpp:64-87) ---- */
+/* one RK4 step with stage Hamiltonians H4 [4][N*N*2] (stage times t, t+h/2, t+h/2, t+h) */
+void oracle_rk4_step_timedep(const oracle_instance* inst, double h, double* y, double* work, const double* H4);
+/* nsteps such steps, stage_h [nsteps][4][N*N*2]; tops as oracle_propagate */
+int64_t oracle_propagate_timedep(const oracle_instance* inst, double h, int64_t nsteps, int64_t observe_steps,
+                                 double* y, double* work, const double* stage_h, double* tops);
 /* ---- adjoint (no reference counterpart: the mathematics of the discrete adjoint) ---- */
 /* out = L^dag mu, the transpose of oracle_rhs as a linear map of the state */
 void oracle_rhs_adjoint(const oracle_instance* inst, const double* mu, double* out);

@@ -102,6 +102,9 @@ def lib():
         L.oracle_adjoint_gradient.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
                                               ctypes.c_int64, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
                                               f64p, ctypes.c_void_p]
+        L.oracle_propagate_timedep.restype = ctypes.c_int64
+        L.oracle_propagate_timedep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+                                               f64p, f64p, f64p, f64p]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_num_threads.argtypes = []
         _lib_handle = L
@@ -564,6 +567,21 @@ class Instance:
         t = tops[: got * self.states * self.states * 2].reshape(got, self.states, self.states, 2)
         return t[..., 0] + 1j * t[..., 1]
 
+    def propagate_timedep(self, y: np.ndarray, h: float, nsteps: int, observe_steps: int,
+                          stage_h: np.ndarray) -> np.ndarray:
+        """propagate() with a time-dependent Hamiltonian: stage_h complex [nsteps, 4, N, N],
+        the Hamiltonian at the stage times t, t + h/2, t + h/2, t + h of each step
+        (the e_field pre-evaluate action, include/heom/e_field.hpp:64-87)."""
+        n = self.states
+        rows = nsteps // observe_steps + 1
+        sh = np.asarray(stage_h, dtype=np.complex128).reshape(nsteps, 4, n, n)
+        sha = np.ascontiguousarray(np.stack([sh.real, sh.imag], -1).reshape(-1))
+        tops = np.zeros(rows * n * n * 2)
+        work = np.empty(5 * self.state_size)
+        got = lib().oracle_propagate_timedep(self.cstruct(), h, nsteps, observe_steps, y, work, sha, tops)
+        t = tops[: got * n * n * 2].reshape(got, n, n, 2)
+        return t[..., 0] + 1j * t[..., 1]
+
     def norms(self, state: np.ndarray, threshold: float = -1.0) -> np.ndarray:
         out = np.zeros(self.matrices)
         lib().oracle_hierarchy_norm(self.states, self.matrices, np.ascontiguousarray(state),
@@ -665,6 +683,93 @@ class Instance:
                 out[k, b] = sum(float(np.sum(np.real(np.conj(grads[key]) * (vals[0][key] - vals[1][key]))))
                                 for key in keys) / (2.0 * step)
         return out
+
+
+def field_stage_hamiltonians(H0: np.ndarray, d_minus: np.ndarray, d_plus: np.ndarray, field, t0: float, h: float,
+                             nsteps: int) -> np.ndarray:
+    """[nsteps, 4, N, N]: H(t) = H0 - E(t) D- - conj(E(t)) D+ (include/heom/e_field.hpp:78)
+    at the RK4 stage times t_n, t_n + h/2, t_n + h/2, t_n + h, t_n = t0 + n h."""
+    c = np.array([0.0, 0.5, 0.5, 1.0])
+    times = t0 + h * (np.arange(nsteps)[:, None] + c[None, :])
+    E = np.vectorize(field, otypes=[np.complex128])(times)
+    return H0[None, None] - E[..., None, None] * d_minus[None, None] - np.conj(E)[..., None, None] * d_plus[None, None]
+
+
+# --------------------------------------------------------------------------------------
+# binary instance file (src/heom/instance.cpp:281-377, include/heom/instance.hpp:31-211)
+# --------------------------------------------------------------------------------------
+
+# packed header: version, hierarchy_top_only, hierarchy_layout, hamiltonian_layout,
+# sites_to_states_mode, time, first_non_plus_id, then seven int64 sizes
+_INSTANCE_HEADER = np.dtype([
+    ("version", "<i4"), ("top_only", "u1"), ("hierarchy_layout", "<i4"), ("hamiltonian_layout", "<i4"),
+    ("sts_mode", "<i4"), ("time", "<f8"), ("first_non_plus", "<i4"),
+    ("states", "<i8"), ("matrices", "<i8"), ("matsubaras", "<i8"), ("baths_per_state", "<i8"),
+    ("baths", "<i8"), ("width", "<i8"), ("depth", "<i8"),
+])
+STS_MODES = {"identity": 0, "with_ground_state": 1, "excited_state_absorption": 2}
+
+
+def instance_file_size(states: int, matrices: int, matsubaras: int, baths: int, width: int, top_only: bool) -> int:
+    """Bytes of an instance file: the 85-byte header, then the buffers in file order."""
+    nn2 = states * states * 2
+    return (_INSTANCE_HEADER.itemsize + 8 * (nn2 if top_only else nn2 * matrices) + 8 * nn2
+            + 4 * 3 * matrices * width + 8 * (2 * matrices + 4 * baths + 3 * baths * matsubaras))
+
+
+def write_instance_file(path: str, inst: Instance, state: np.ndarray, top_only: bool = False, time: float = 0.0,
+                        mode: str = "identity") -> None:
+    """instance::write_file (src/heom/instance.cpp:333-377) for an oracle instance."""
+    h = np.zeros((), dtype=_INSTANCE_HEADER)
+    h["top_only"] = int(bool(top_only))
+    h["sts_mode"] = STS_MODES[mode]
+    h["time"] = time
+    h["first_non_plus"] = inst.first_non_plus
+    for k, v in (("states", inst.states), ("matrices", inst.matrices), ("matsubaras", inst.matsubaras),
+                 ("baths_per_state", inst.baths_max_per_state), ("baths", inst.baths), ("width", inst.width),
+                 ("depth", inst.depth)):
+        h[k] = v
+    st = np.asarray(state, dtype="<f8").reshape(-1)
+    nn2 = inst.states * inst.states * 2
+    with open(path, "wb") as f:
+        f.write(h.tobytes())
+        f.write(st[:nn2].tobytes() if top_only else st.tobytes())
+        for a, t in ((inst.hamiltonian, "<f8"), (inst.tuples, "<i4"), (inst.plus, "<i4"), (inst.minus, "<i4"),
+                     (inst.pf_same_sum, "<f8"), (inst.pf_same, "<f8"), (inst.pf_circ, "<f8"), (inst.cbk, "<f8"),
+                     (inst.cbk_sqrt, "<f8")):
+            f.write(np.ascontiguousarray(a, dtype=t).tobytes())
+
+
+def read_instance_file(path: str) -> dict:
+    """instance::read_file (src/heom/instance.cpp:281-330): header fields and buffers
+    (hierarchy as written: the top ADO only for a top-only file)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    h = np.frombuffer(data, dtype=_INSTANCE_HEADER, count=1)[0]
+    out = {k: h[k].item() for k in _INSTANCE_HEADER.names}
+    out["top_only"] = bool(out["top_only"])
+    N, M, K, B, W = (out[k] for k in ("states", "matrices", "matsubaras", "baths", "width"))
+    if len(data) != instance_file_size(N, M, K, B, W, out["top_only"]):
+        raise ValueError("instance file has the wrong size for its header")
+    off = _INSTANCE_HEADER.itemsize
+    nn2 = N * N * 2
+
+    def take(n, t):
+        nonlocal off
+        a = np.frombuffer(data, dtype=t, count=n, offset=off).copy()
+        off += a.nbytes
+        return a
+
+    out["hierarchy"] = take(nn2 if out["top_only"] else nn2 * M, "<f8")
+    out["hamiltonian"] = take(nn2, "<f8")
+    for k in ("tuples", "plus", "minus"):
+        out[k] = take(M * W, "<i4")
+    out["pf_same_sum"] = take(2 * M, "<f8")
+    out["pf_same"] = take(2 * B, "<f8")
+    out["pf_circ"] = take(2 * B, "<f8")
+    out["cbk"] = take(2 * B * K, "<f8")
+    out["cbk_sqrt"] = take(B * K, "<f8")
+    return out
 
 
 def trace(d: np.ndarray | None, top: np.ndarray) -> complex:
