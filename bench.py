@@ -14,7 +14,7 @@ Prints one JSON line with ADO-RK4-steps/s, the roofline of the stage kernel and 
 CPU oracle timed on the host cores beside it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--no-cpu-baseline]
-                  [--no-extras | --extras precision,ensemble,adjoint]
+                  [--no-extras | --extras precision,ensemble,adjoint,filtering,field,tss]
 """
 from __future__ import annotations
 
@@ -42,9 +42,9 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the FP32-storage run of the same workload and the batched ensemble")
-    p.add_argument("--extras", default="precision,ensemble,adjoint,filtering,field",
+    p.add_argument("--extras", default="precision,ensemble,adjoint,filtering,field,tss",
                    help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, "
-                        "adjoint, filtering (compaction), field (time-dependent H)")
+                        "adjoint, filtering (compaction), field (time-dependent H), tss (thermal state search)")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--copy-mib", type=int, default=0,
@@ -338,6 +338,29 @@ def field_run(host, steps: int = 40) -> dict:
             "ms_per_step_field": t_field * 1e3, "ms_per_step_constant_h": t_plain * 1e3}
 
 
+def tss_run(host, iterations: int = 20) -> dict:
+    """Thermal state search iterations (heom_tss_step: one f_TSS evaluation by the stage
+    kernel plus the per-ADO eigenbasis transform) on the same hierarchy, timed on the
+    host around a synchronised run of `iterations` iterations."""
+    import time
+
+    h = host.create_tss_handle()
+    h.tss_step(2)
+    h.sync()
+    t0 = time.perf_counter()
+    h.tss_step(iterations)
+    h.sync()
+    per = (time.perf_counter() - t0) / iterations
+    top = h.get_top()
+    h.close()
+    nel = host.states * host.states
+    transform_bytes = 2.0 * host.matrices * nel * 16
+    return {"workload": f"thermal state search on the {host.matrices}-ADO hierarchy, {iterations} iterations",
+            "ms_per_iteration": per * 1e3, "ado_iterations_per_s": host.matrices / per,
+            "transform_alg_bytes": transform_bytes,
+            "top_trace_minus_1": float(abs(np.trace(top) - 1.0))}
+
+
 def ensemble_run(members: int) -> dict:
     """The 512-realisation static-disorder ensemble (7 sites, depths 3-6, populations
     plus xx/yy/zz linear absorption, 2048 steps of 2 fs), batched, FP64 and FP32."""
@@ -468,6 +491,8 @@ def main():
         out["filtering"] = filtering_run(host)
     if "field" in extras:
         out["field"] = field_run(host)
+    if "tss" in extras:
+        out["tss"] = tss_run(host)
     if "ensemble" in extras:
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value

@@ -387,6 +387,7 @@ int heom_rhs_adjoint(heom_handle* h, double* out)
 {
     return guard([&] {
         need(h);
+        reject_tss(h, "the adjoint");
         if (!out)
             raise(HEOM_E_INVALID, "out is NULL");
         check_adjoint_supported(h);
@@ -401,6 +402,7 @@ int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t obse
 {
     return guard([&] {
         need(h);
+        reject_tss(h, "the adjoint");
         if (!ops || !observed)
             raise(HEOM_E_INVALID, "ops and observed must not be NULL");
         if (nsteps < 0 || observe_steps < 1 || checkpoint_steps < 1)

@@ -93,6 +93,13 @@ struct AdjointState {
     DevBuf<double2> src;                         // [rows][count][nel] adjoint sources conj(w_r) O^dag
 };
 
+// The thermal state search (tss.hip): the constants of the per-ADO transform.
+struct TssState {
+    DevBuf<double2> c, ct;  // C and C^T [nel]
+    DevBuf<double2> w;      // (i/hbar)(E_i - E_j) [nel]
+    DevBuf<double2> epsm;   // eps - S_u per slot [Mp]
+};
+
 }  // namespace heom_rt
 
 struct heom_handle {
@@ -162,6 +169,8 @@ struct heom_handle {
 
     // the recorded forward sweep and buffers of the adjoint (adjoint.hip)
     std::unique_ptr<heom_rt::AdjointState> adj;
+    // set on a thermal-state-search handle (heom_create_tss), whose operator is f_TSS
+    std::unique_ptr<heom_rt::TssState> tss;
 
     bool fp32() const { return prec == HEOM_PRECISION_FP32; }
     size_t cbytes() const { return fp32() ? sizeof(float2) : sizeof(double2); }
@@ -195,6 +204,10 @@ inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + thre
 // by stage 4), yout the stage output; adjoint = true applies L^dag instead of L
 void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void* y0, void* pbuf, void* yout,
                           double c1, bool reverse, bool adjoint);
+// builds the device layout of count instances into a fresh handle (heom_create_batch)
+void create_batch(heom_handle* h, const heom_desc* d, int count);
+// raises HEOM_E_UNSUPPORTED for `what` on a thermal-state-search handle
+void reject_tss(heom_handle* h, const char* what);
 // nsteps RK4 steps of the handle's state (through the captured step graph when enabled)
 void rk4_steps(heom_handle* h, double dt, int64_t nsteps);
 // the tops of all instances [count][nel] into dst (device)

@@ -536,6 +536,7 @@ void drop_graph(heom_handle* h)
 // launches only.
 void rk4_steps(heom_handle* h, double dt, int64_t nsteps)
 {
+    reject_tss(h, "RK4 propagation");
     if (!h->use_graphs || nsteps < 2) {
         for (int64_t s = 0; s < nsteps; ++s)
             rk4_step_launches(h, dt);
@@ -579,6 +580,7 @@ void refold_tables(heom_handle* h)
 // H [count][N*N*2] in J -> H / hbar on the device
 void set_hamiltonians(heom_handle* h, const double* H)
 {
+    reject_tss(h, "setting the Hamiltonian");
     const size_t n = (size_t)h->count * h->nel;
     std::vector<double2> hs(n);
     std::vector<float2> hs32(n);
@@ -1329,6 +1331,7 @@ int heom_propagate_timedep(heom_handle* h, double dt, int64_t nsteps, int64_t ob
 {
     return guard([&] {
         need(h);
+        reject_tss(h, "heom_propagate_timedep");
         if (!tops || !stage_h)
             raise(HEOM_E_INVALID, "tops and stage_hamiltonians must not be NULL");
         if (nsteps < 0 || observe_steps < 1)
@@ -1552,6 +1555,7 @@ int heom_compact(heom_handle* h, int64_t* active)
 {
     return guard([&] {
         need(h);
+        reject_tss(h, "heom_compact");
         if (h->fp32())
             raise(HEOM_E_UNSUPPORTED, "compaction needs FP64 storage");
         const int W = h->W, nel = h->nel;
@@ -1659,6 +1663,7 @@ int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int3
 {
     return guard([&] {
         need(h);
+        reject_tss(h, "heom_autotune");
         if (steps < 1)
             raise(HEOM_E_INVALID, "steps must be >= 1");
         if (capacity > 0 && !results)

@@ -39,7 +39,7 @@ EXPORTED = [
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
-    "heom_propagate_timedep",
+    "heom_propagate_timedep", "heom_create_tss", "heom_tss_step",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -133,6 +133,8 @@ def load():
                               ctypes.POINTER(ctypes.c_int32)],
             "heom_compact": [vp, ctypes.POINTER(i64)],
             "heom_propagate_timedep": [vp, dbl, i64, i64, vp, vp],
+            "heom_create_tss": [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
+            "heom_tss_step": [vp, i64],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -246,6 +248,42 @@ class Handle:
         """A batch handle over several instances with the same states and baths."""
         made = [_make_desc(precision=precision, **_instance_arrays(i)) for i in insts]
         return cls._create([d for d, _ in made], [k for _, k in made])
+
+    @classmethod
+    def thermal_state_search(cls, desc: Desc, c, eigenvalues, epsilon, keepalive=None) -> "Handle":
+        """heom_create_tss: a search handle for one instance (desc FP64), with C (the
+        transposed eigenvectors of H), the eigenvalues (J) and epsilon."""
+        self = cls.__new__(cls)
+        N = int(desc.states)
+        cc = _f64(np.stack([np.real(c), np.imag(c)], -1).reshape(-1))
+        ev = _f64(np.stack([np.real(eigenvalues), np.imag(eigenvalues)], -1).reshape(-1))
+        if cc.size != 2 * N * N or ev.size != 2 * N:
+            raise ValueError("c must be N x N and eigenvalues of length N")
+        eps = np.array([complex(epsilon).real, complex(epsilon).imag])
+        h = ctypes.c_void_p()
+        _check(load().heom_create_tss(ctypes.byref(desc), cc.ctypes.data, ev.ctypes.data, eps.ctypes.data,
+                                      ctypes.byref(h)))
+        self._h = h
+        self.count = 1
+        self.states = N
+        self.instance_matrices = [int(desc.matrices)]
+        self.matrices = int(desc.matrices)
+        self.width = desc.width
+        self.baths = desc.baths
+        self.matsubaras = desc.matsubaras
+        self.precision = desc.precision
+        self._keep = keepalive
+        return self
+
+    @classmethod
+    def thermal_state_search_from_instance(cls, inst, setup: dict) -> "Handle":
+        """A search handle over an instance's arrays with setup's c, eigenvalues, epsilon."""
+        desc, keep = _make_desc(precision=HEOM_PRECISION_FP64, **_instance_arrays(inst))
+        return cls.thermal_state_search(desc, setup["c"], setup["eigenvalues"], setup["epsilon"], keep)
+
+    def tss_step(self, n: int = 1) -> None:
+        """n iterations of the thermal state search (heom_tss_step)."""
+        _check(load().heom_tss_step(self._h, int(n)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -470,12 +508,14 @@ class Handle:
 
 TASK_POPULATION_DYNAMICS = 0
 TASK_LINEAR_ABSORPTION = 1
+TASK_THERMAL_STATE_SEARCH = 2
 
 HOST_EXPORTED = [
     "heom_host_open", "heom_host_close", "heom_host_last_error", "heom_host_get_info", "heom_host_desc",
     "heom_host_top", "heom_host_cbk_sqrt", "heom_host_mask", "heom_host_dipoles", "heom_host_partitions",
     "heom_host_set_parameters", "heom_host_get_parameters", "heom_host_parameter_gradient",
-    "heom_host_write_instance", "heom_host_read_instance_state",
+    "heom_host_write_instance", "heom_host_read_instance_state", "heom_host_tss_setup",
+    "heom_host_symmetric_eigen",
 ]
 
 
@@ -484,7 +524,8 @@ class HostInfo(ctypes.Structure):
                 ("matsubaras", ctypes.c_int64), ("width", ctypes.c_int64), ("depth", ctypes.c_int64),
                 ("matrices", ctypes.c_int64), ("first_non_plus", ctypes.c_int64),
                 ("step_size", ctypes.c_double), ("steps", ctypes.c_int64),
-                ("observe_steps", ctypes.c_int64), ("tensor_components", ctypes.c_int64)]
+                ("observe_steps", ctypes.c_int64), ("tensor_components", ctypes.c_int64),
+                ("tss_delta", ctypes.c_double), ("tss_max_steps", ctypes.c_int64)]
 
 
 class ConfigError(ValueError):
@@ -516,6 +557,8 @@ def load_host():
             "heom_host_get_parameters": ([vp, vp, vp, vp, vp], ctypes.c_int),
             "heom_host_parameter_gradient": ([vp, ctypes.POINTER(Gradients), vp, vp, vp, vp], ctypes.c_int),
             "heom_host_write_instance": ([vp, ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+            "heom_host_tss_setup": ([vp, vp, vp, vp, vp], ctypes.c_int),
+            "heom_host_symmetric_eigen": ([ctypes.c_int, vp, vp, vp], ctypes.c_int),
             "heom_host_read_instance_state": ([vp, ctypes.c_char_p, vp, vp], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
@@ -534,6 +577,18 @@ def _host_check(rc: int) -> None:
         if rc == -5:
             raise ConfigError(msg)
         raise HeomError(rc, msg)
+
+
+def symmetric_eigen(a) -> tuple:
+    """(values, vectors as columns) of a real symmetric matrix: the host library's Jacobi
+    routine of the thermal state search set-up."""
+    m = _f64(a)
+    n = int(np.sqrt(m.size))
+    if n * n != m.size:
+        raise ValueError("a must be square")
+    w, v = np.zeros(n), np.zeros(n * n)
+    _host_check(load_host().heom_host_symmetric_eigen(n, m.ctypes.data, w.ctypes.data, v.ctypes.data))
+    return w, v.reshape(n, n)
 
 
 def partitions(n: int, s: int) -> np.ndarray:
@@ -665,6 +720,26 @@ class HostInstance:
                                                              dl.ctypes.data, dg.ctypes.data, do.ctypes.data))
         dH = dH.reshape(self.sites, self.sites, 2)
         return {"hamiltonian": dH[..., 0] + 1j * dH[..., 1], "lambda": dl, "gamma": dg, "omega": do}
+
+    def tss_setup(self) -> dict:
+        """heom_host_tss_setup: C, the eigenvalues (J), the Boltzmann top and epsilon."""
+        N = self.states
+        c, ev, top, eps = np.zeros(2 * N * N), np.zeros(2 * N), np.zeros(2 * N * N), np.zeros(2)
+        _host_check(load_host().heom_host_tss_setup(self._p, c.ctypes.data, ev.ctypes.data, top.ctypes.data,
+                                                    eps.ctypes.data))
+
+        def cx(a, shape):
+            return (a[0::2] + 1j * a[1::2]).reshape(shape)
+        return {"c": cx(c, (N, N)), "eigenvalues": cx(ev, (N,)), "top": cx(top, (N, N)),
+                "epsilon": complex(eps[0], eps[1])}
+
+    def create_tss_handle(self) -> Handle:
+        """heom_create_tss on the instance with its heom_host_tss_setup constants, the
+        Boltzmann top loaded."""
+        s = self.tss_setup()
+        h = Handle.thermal_state_search(self.desc(), s["c"], s["eigenvalues"], s["epsilon"], keepalive=self)
+        h.set_top(s["top"])
+        return h
 
     def write_instance(self, path: str, state=None, top_only: bool = False, time: float = 0.0) -> None:
         """Write the reference's binary instance file with `state` (reference layout), or

@@ -191,6 +191,7 @@ const char* kCommonKeys[] = {
     "baths.matsubaras", "baths.temperature",
 };
 const char* kPopulationKeys[] = {"population_dynamics.rho_init"};
+const char* kThermalKeys[] = {"thermal_state_search.delta", "thermal_state_search.max_steps"};
 const char* kDipoleKeys[] = {"dipole.directions", "dipole.strengths", "dipole.centers",
                              "dipole.tensor_prefactors", "dipole.tensor_components"};
 
@@ -243,15 +244,19 @@ config::config(const std::string& path, program_task task) : task_(task)
     auto it = kv.find("program.task");
     if (it == kv.end())
         throw config_error("the option 'program.task' is required but missing");
-    const std::string want = task == program_task::population_dynamics ? "population_dynamics" : "linear_absorption";
+    const std::string want = task == program_task::population_dynamics    ? "population_dynamics"
+                             : task == program_task::linear_absorption    ? "linear_absorption"
+                                                                          : "thermal_state_search";
     if (it->second != want)
         throw config_error("wrong program_task for this application");
 
     std::set<std::string> allowed(std::begin(kCommonKeys), std::end(kCommonKeys));
     if (task == program_task::population_dynamics)
         allowed.insert(std::begin(kPopulationKeys), std::end(kPopulationKeys));
-    else
+    else if (task == program_task::linear_absorption)
         allowed.insert(std::begin(kDipoleKeys), std::end(kDipoleKeys));
+    else
+        allowed.insert(std::begin(kThermalKeys), std::end(kThermalKeys));
     for (auto& [k, v] : kv)
         if (!allowed.count(k))
             throw config_error("unrecognised option '" + k + "'");
@@ -300,6 +305,9 @@ config::config(const std::string& path, program_task task) : task_(task)
     if (task == program_task::population_dynamics) {
         rho_init_ = to_matrix<complex_t>(lit("population_dynamics.rho_init"), "population_dynamics.rho_init",
                                          to_complex);
+    } else if (task == program_task::thermal_state_search) {
+        tss_delta_ = to_real(lit("thermal_state_search.delta"), "thermal_state_search.delta");
+        tss_max_steps_ = to_int(lit("thermal_state_search.max_steps"), "thermal_state_search.max_steps");
     } else {
         auto vecs = [&](const char* key) {
             const literal l = lit(key);
@@ -388,6 +396,10 @@ void config::check() const
     if (task_ == program_task::population_dynamics) {
         if (!((int_t)rho_init_.rows == sites_ && (int_t)rho_init_.cols == sites_))
             throw config_error("population_dynamics.rho_init must be a system.sites by system.sites matrix");
+    } else if (task_ == program_task::thermal_state_search) {
+        // thermal_state_search_config::check_final (thermal_state_search_config.cpp:31-39)
+        if (step_size_ != 1.0)
+            throw config_error("solver.step_size_ must equal 1.0");
     } else {
         if (dipole_directions_.size() != dipole_strengths_.size() || dipole_directions_.size() != dipole_centers_.size())
             throw config_error("dipole.directions, dipole.strengths and dipole.centers must have the same size");

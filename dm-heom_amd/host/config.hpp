@@ -62,7 +62,7 @@ literal parse_literal(const std::string& s);
 
 // ---- config ---------------------------------------------------------------------
 
-enum class program_task { population_dynamics, linear_absorption };
+enum class program_task { population_dynamics, linear_absorption, thermal_state_search };
 
 struct observation_entry {
     std::string type;
@@ -108,6 +108,9 @@ public:
     real_t baths_temperature() const { return temperature_; }
     // [population_dynamics]
     const complex_matrix& population_dynamics_rho_init() const { return rho_init_; }
+    // [thermal_state_search] (thermal_state_search_config.cpp:19-23)
+    real_t thermal_state_search_delta() const { return tss_delta_; }
+    int_t thermal_state_search_max_steps() const { return tss_max_steps_; }
     // [dipole]
     const std::vector<std::vector<real_t>>& dipole_directions() const { return dipole_directions_; }
     const std::vector<real_t>& dipole_strengths() const { return dipole_strengths_; }
@@ -140,6 +143,8 @@ private:
     int_t matsubaras_ = 0;
     real_t temperature_ = 0.0;
     complex_matrix rho_init_;
+    real_t tss_delta_ = 0.0;
+    int_t tss_max_steps_ = 0;
     std::vector<std::vector<real_t>> dipole_directions_, dipole_centers_;
     std::vector<real_t> dipole_strengths_;
     std::vector<complex_t> tensor_prefactors_;

@@ -9,6 +9,7 @@
 #include "constants.hpp"
 #include "heom_host.h"
 #include "hierarchy.hpp"
+#include "thermal_state.hpp"
 #include "solver.hpp"
 
 struct heom_host_instance {
@@ -59,23 +60,28 @@ int heom_host_open(const char* path, int task, int64_t depth_override, heom_host
     return host_guard([&] {
         if (!path)
             throw std::invalid_argument("config path is NULL");
-        if (task != HEOM_TASK_POPULATION_DYNAMICS && task != HEOM_TASK_LINEAR_ABSORPTION)
+        if (task != HEOM_TASK_POPULATION_DYNAMICS && task != HEOM_TASK_LINEAR_ABSORPTION &&
+            task != HEOM_TASK_THERMAL_STATE_SEARCH)
             throw std::invalid_argument("unknown task");
         auto p = std::make_unique<heom_host_instance>();
-        const auto t = task == HEOM_TASK_POPULATION_DYNAMICS ? heom::program_task::population_dynamics
-                                                             : heom::program_task::linear_absorption;
+        const auto t = task == HEOM_TASK_POPULATION_DYNAMICS  ? heom::program_task::population_dynamics
+                       : task == HEOM_TASK_LINEAR_ABSORPTION ? heom::program_task::linear_absorption
+                                                              : heom::program_task::thermal_state_search;
         p->cfg = std::make_unique<heom::config>(path, t);
         if (depth_override > 0)
             p->cfg->set_ado_depth(depth_override);
         p->graph = std::make_unique<heom::hierarchy_graph>(p->cfg->baths_number(), p->cfg->baths_matsubaras(),
                                                            p->cfg->system_ado_depth());
-        const auto mode = task == HEOM_TASK_POPULATION_DYNAMICS ? heom::sites_to_states_mode::identity
-                                                                : heom::sites_to_states_mode::with_ground_state;
+        // the search runs on the one-exciton states (app_thermal_state_search.cpp:60)
+        const auto mode = task == HEOM_TASK_LINEAR_ABSORPTION ? heom::sites_to_states_mode::with_ground_state
+                                                              : heom::sites_to_states_mode::identity;
         p->inst = std::make_unique<heom::instance>(*p->cfg, mode, *p->graph);
         if (task == HEOM_TASK_POPULATION_DYNAMICS)
             p->inst->set_hierarchy_top(p->cfg->population_dynamics_rho_init().data.data());
-        else
+        else if (task == HEOM_TASK_LINEAR_ABSORPTION)
             p->inst->set_hierarchy_top();
+        else
+            p->inst->set_hierarchy_top(heom::thermal_state_setup(*p->inst, p->cfg->baths_temperature()).top.data());
         *out = p.release();
     });
 }
@@ -106,6 +112,8 @@ int heom_host_get_info(const heom_host_instance* p, heom_host_info* info)
         info->steps = p->cfg->solver_steps();
         info->observe_steps = p->cfg->program_observe_steps();
         info->tensor_components = (int64_t)p->cfg->dipole_tensor_prefactors().size();
+        info->tss_delta = p->cfg->thermal_state_search_delta();
+        info->tss_max_steps = p->cfg->thermal_state_search_max_steps();
     });
 }
 
@@ -272,6 +280,27 @@ int heom_host_parameter_gradient(const heom_host_instance* p, const heom_gradien
     });
 }
 
+int heom_host_tss_setup(const heom_host_instance* p, double* c, double* eigenvalues, double* top, double* epsilon)
+{
+    return host_guard([&] {
+        need(p);
+        const auto s = heom::thermal_state_setup(*p->inst, p->cfg->baths_temperature());
+        if (c)
+            std::copy(s.c.begin(), s.c.end(), c);
+        if (eigenvalues)
+            std::copy(s.eigenvalues.begin(), s.eigenvalues.end(), eigenvalues);
+        if (top)
+            for (std::size_t e = 0; e < s.top.size(); ++e) {
+                top[2 * e] = s.top[e].real();
+                top[2 * e + 1] = s.top[e].imag();
+            }
+        if (epsilon) {
+            epsilon[0] = s.epsilon.real();
+            epsilon[1] = s.epsilon.imag();
+        }
+    });
+}
+
 int heom_host_write_instance(const heom_host_instance* p, const char* path, const double* state, int top_only,
                              double time)
 {
@@ -291,6 +320,22 @@ int heom_host_read_instance_state(const heom_host_instance* p, const char* path,
         if (!path || !state)
             throw std::invalid_argument("path and state must not be NULL");
         p->inst->read_state(path, state, time);
+    });
+}
+
+int heom_host_symmetric_eigen(int n, const double* a, double* values, double* vectors)
+{
+    return host_guard([&] {
+        if (n < 1 || !a || !values || !vectors)
+            throw std::invalid_argument("n must be >= 1 and the arrays non-NULL");
+        std::vector<double> m(a, a + (std::size_t)n * n), e, v;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j)
+                if (m[(std::size_t)i * n + j] != m[(std::size_t)j * n + i])
+                    throw std::invalid_argument("the matrix is not symmetric");
+        heom::symmetric_eigen(n, m, e, v);
+        std::copy(e.begin(), e.end(), values);
+        std::copy(v.begin(), v.end(), vectors);
     });
 }
 

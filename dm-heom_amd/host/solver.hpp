@@ -25,6 +25,7 @@
 #include "config.hpp"
 #include "heom_hip.h"
 #include "hierarchy.hpp"
+#include "thermal_state.hpp"
 
 namespace heom {
 
@@ -115,6 +116,76 @@ private:
     real_t step_size_;
     int_t step_count_ = 0;
     real_t time_ = 0.0;
+};
+
+// thermal_state_search_solver (dm-heom/include/heom/thermal_state_search_solver.hpp:31-376)
+// over heom_create_tss: the set-up of initialize() (:124-209) on the host, step_forward
+// (:267-299) and test_convergence (:302-325) over the device state.
+class thermal_state_search_solver {
+public:
+    thermal_state_search_solver(const config& cfg, instance& inst)
+        : inst_(inst), setup_(thermal_state_setup(inst, cfg.baths_temperature()))
+    {
+        const heom_desc d = inst.desc();
+        const double eps[2] = {setup_.epsilon.real(), setup_.epsilon.imag()};
+        hip_call(heom_create_tss(&d, setup_.c.data(), setup_.eigenvalues.data(), eps, &h_));
+        inst.set_hierarchy_top(setup_.top.data());
+        const std::size_t nn = (std::size_t)(inst.states() * inst.states());
+        last_top_.resize(nn);
+        current_top_.resize(nn);
+        reset();
+    }
+    ~thermal_state_search_solver()
+    {
+        if (h_)
+            heom_destroy(h_);
+    }
+    thermal_state_search_solver(const thermal_state_search_solver&) = delete;
+    thermal_state_search_solver& operator=(const thermal_state_search_solver&) = delete;
+
+    void reset()
+    {
+        hip_call(heom_clear_state(h_));
+        hip_call(heom_set_top(h_, inst_.top().data()));
+        step_count_ = 0;
+    }
+    // n iterations; the top before them is kept for test_convergence
+    void step_forward(int_t n)
+    {
+        hierarchy_top(last_top_.data());
+        hip_call(heom_tss_step(h_, n));
+        step_count_ += n;
+    }
+    // |sum |last top| - sum |current top|| < delta (:302-325)
+    bool test_convergence(real_t delta)
+    {
+        hierarchy_top(current_top_.data());
+        auto norm1 = [](const std::vector<complex_t>& m) {
+            real_t v = 0.0;
+            for (auto& x : m)
+                v += std::abs(x);
+            return v;
+        };
+        return std::abs(norm1(last_top_) - norm1(current_top_)) < delta;
+    }
+
+    void hierarchy_top(complex_t* buffer) { hip_call(heom_get_top(h_, reinterpret_cast<double*>(buffer))); }
+    void hierarchy(complex_t* buffer) { hip_call(heom_get_state(h_, reinterpret_cast<double*>(buffer))); }
+    void hierarchy_norm(real_t* buffer, real_t threshold = -1.0)
+    {
+        hip_call(heom_hierarchy_norm(h_, threshold, buffer));
+    }
+    void update_hierarchy_mask(const std::uint8_t* mask) { hip_call(heom_set_mask(h_, mask)); }
+    int_t step_count() const { return step_count_; }
+    const thermal_setup& setup() const { return setup_; }
+    heom_handle* handle() { return h_; }
+
+private:
+    instance& inst_;
+    thermal_setup setup_;
+    heom_handle* h_ = nullptr;
+    int_t step_count_ = 0;
+    std::vector<complex_t> last_top_, current_top_;
 };
 
 // ---- observation output ----------------------------------------------------------

@@ -170,6 +170,21 @@ int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_
  * stage_hamiltonians [nsteps][4][count][N*N*2] (J).  Steps are not graph-captured. */
 int heom_propagate_timedep(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                            const double* stage_hamiltonians, double* tops /* [rows][count][N*N*2] */);
+/* Thermal state search (dm-heom/include/heom/thermal_state_search_solver.hpp:124-300).
+ * heom_create_tss builds a search handle for one instance: desc as for heom_create
+ * (FP64; its Hamiltonian is not used, the search replaces it by the pseudo-Hamiltonian
+ * with epsilon in every element, :199-209), c = C [N*N*2] (the transposed
+ * eigenvectors of H, :160-161), eigenvalues [N*2] of H in J and epsilon [2].  One
+ * heom_tss_step iteration is the reference's step_forward(1) (:267-299):
+ *   k = f_TSS(y) (heom_ode.cl with HEOM_TSS_EQUATION),  k <- C k C^T,
+ *   y_u = k_u / (i (E_i - E_j) / hbar + epsilon - pf_same_sum_u)  (thermal_state_search.cl:36-48),
+ *   y <- C^T y C.
+ * Load the start (the Boltzmann top, heom_host_tss_setup) with heom_set_top; state,
+ * top, norms, mask and heom_rhs (= f_TSS) work as on any handle; RK4 propagation,
+ * Hamiltonian updates, compaction and the adjoint are refused on a search handle. */
+int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenvalues, const double* epsilon,
+                    heom_handle** out);
+int heom_tss_step(heom_handle* handle, int64_t n);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
  * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */
 int heom_mmult_left(heom_handle* handle, const double* mx /* [count][N*N*2] */);

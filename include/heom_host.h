@@ -25,7 +25,11 @@ extern "C" {
 
 enum heom_host_status { HEOM_HOST_OK = 0, HEOM_HOST_E_INVALID = -1, HEOM_HOST_E_CONFIG = -5 };
 
-enum heom_host_task { HEOM_TASK_POPULATION_DYNAMICS = 0, HEOM_TASK_LINEAR_ABSORPTION = 1 };
+enum heom_host_task {
+    HEOM_TASK_POPULATION_DYNAMICS = 0,
+    HEOM_TASK_LINEAR_ABSORPTION = 1,
+    HEOM_TASK_THERMAL_STATE_SEARCH = 2
+};
 
 typedef struct heom_host_instance heom_host_instance;
 
@@ -42,6 +46,8 @@ typedef struct heom_host_info {
     int64_t steps;
     int64_t observe_steps;
     int64_t tensor_components; /* linear absorption: dipole tensor components */
+    double tss_delta;          /* thermal state search: thermal_state_search.delta */
+    int64_t tss_max_steps;     /* thermal state search: thermal_state_search.max_steps */
 } heom_host_info;
 
 /* task: enum heom_host_task; depth_override > 0 replaces system.ado_depth */
@@ -75,6 +81,13 @@ int heom_host_get_parameters(const heom_host_instance* inst, double* hamiltonian
  * d_gamma, d_omega [B].  Any output may be NULL. */
 int heom_host_parameter_gradient(const heom_host_instance* inst, const heom_gradients* grads,
                                  double* d_hamiltonian, double* d_lambda, double* d_gamma, double* d_omega);
+/* thermal_state_search_solver::initialize (dm-heom/include/heom/thermal_state_search_solver.hpp:124-209)
+ * for the instance's current Hamiltonian, which must be real symmetric: c = C [N*N*2]
+ * (the transposed eigenvectors), eigenvalues [N*2] (J), top [N*N*2] the Boltzmann state
+ * the search starts from, epsilon [2] the pseudo-Hamiltonian's constant; outputs may be
+ * NULL.  An instance opened with HEOM_TASK_THERMAL_STATE_SEARCH already holds this top. */
+int heom_host_tss_setup(const heom_host_instance* inst, double* c, double* eigenvalues, double* top,
+                        double* epsilon);
 /* The reference's binary instance format (dm-heom/src/heom/instance.cpp:281-377): a
  * checkpoint of a hierarchy with every kernel-input array.  write stores this instance
  * with the state [M*N*N*2] (state NULL or top_only: the top ADO only) and a time;
@@ -84,6 +97,11 @@ int heom_host_parameter_gradient(const heom_host_instance* inst, const heom_grad
 int heom_host_write_instance(const heom_host_instance* inst, const char* path, const double* state, int top_only,
                              double time);
 int heom_host_read_instance_state(const heom_host_instance* inst, const char* path, double* state, double* time);
+/* eigen decomposition of a real symmetric n x n matrix a (row-major) by cyclic Jacobi
+ * rotations, the routine heom_host_tss_setup uses (the reference calls Eigen's
+ * ComplexEigenSolver, include/heom/eigen_decomposition.hpp:17-36): values [n] and the
+ * orthonormal eigenvectors as the columns of vectors [n*n] (row-major) */
+int heom_host_symmetric_eigen(int n, const double* a, double* values, double* vectors);
 /* integer partitions of n into at most s parts in the reference order; writes
  * min(count, capacity) rows of s entries, returns count (negative on error) */
 int64_t heom_host_partitions(int n, int s, int32_t* out, int64_t capacity);

@@ -372,7 +372,7 @@ int oracle_graph_build(int width, int depth, int32_t* tuples, int32_t* plus, int
         map_insert(&mp, u);
 
     int err = 0;
-#pragma omp parallel
+#pragma omp parallel if (nodes > 4096)
     {
         int32_t* probe = (int32_t*)malloc(sizeof(int32_t) * (size_t)width);
 #pragma omp for schedule(static)
@@ -460,7 +460,7 @@ void oracle_precompute(int baths, int matsubaras, const double* lambda, const do
     }
 
     /* S_n = -sum_{b,k} n_bk nu_bk in long double (instance.cpp:227-262) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (matrices > 1024)
     for (int64_t id = 0; id < matrices; ++id) {
         long double zr = 0.0L, zi = 0.0L;
         for (int b = 0; b < baths; ++b) {
@@ -488,7 +488,8 @@ void oracle_precompute(int baths, int matsubaras, const double* lambda, const do
 /* RHS (cl/heom/heom_ode.cl:24-451)                                          */
 /* ======================================================================== */
 
-void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
+/* tss = 1: heom_ode.cl compiled with HEOM_TSS_EQUATION (:116-121, :130-136) */
+static void rhs_eval(const oracle_instance* inst, const double* in, double* out, int tss)
 {
     const int N = inst->states;
     const int K = inst->matsubaras;
@@ -498,7 +499,7 @@ void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
     const double HB = ORACLE_HBAR;
     const double* H = inst->hamiltonian;
 
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel for schedule(dynamic, 64) if (M * N * N > 16384)
     for (int64_t id = 0; id < M; ++id) {
         const double* s = in + id * N * N * 2;
         double* o = out + id * N * N * 2;
@@ -507,9 +508,14 @@ void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
                 const double s_r = s[(i * N + j) * 2];
                 const double s_i = s[(i * N + j) * 2 + 1];
 
-                /* commutator (i/ħ)(σH - Hσ) with separate temporaries (:78-114) */
+                /* commutator (i/ħ)(σH - Hσ) with separate temporaries (:78-114); for the
+                 * thermal state search the elementwise product σ_ij H_ij (:116-121) */
                 double com_r = 0.0, com_i = 0.0;
-                for (int k = 0; k < N; ++k) {
+                if (tss) {
+                    com_r = s_r * H[(i * N + j) * 2] - s_i * H[(i * N + j) * 2 + 1];
+                    com_i = s_r * H[(i * N + j) * 2 + 1] + s_i * H[(i * N + j) * 2];
+                }
+                for (int k = 0; k < N && !tss; ++k) {
                     double sh_r = s[(i * N + k) * 2] * H[(k * N + j) * 2];
                     sh_r -= s[(i * N + k) * 2 + 1] * H[(k * N + j) * 2 + 1];
                     double sh_i = s[(i * N + k) * 2] * H[(k * N + j) * 2 + 1];
@@ -521,15 +527,16 @@ void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
                     com_r += sh_r - hs_r;
                     com_i += sh_i - hs_i;
                 }
-                {
+                if (!tss) {
                     double t = com_r;
                     com_r = -com_i / HB;
                     com_i = t / HB;
                 }
 
-                /* same term: S_n σ_ij and -Δ_b for the baths of i and of j (:126-169) */
-                const double pss_r = inst->pf_same_sum[2 * id];
-                const double pss_i = inst->pf_same_sum[2 * id + 1];
+                /* same term: S_n σ_ij (not for the thermal state search) and -Δ_b for the
+                 * baths of i and of j (:126-169) */
+                const double pss_r = tss ? 0.0 : inst->pf_same_sum[2 * id];
+                const double pss_i = tss ? 0.0 : inst->pf_same_sum[2 * id + 1];
                 double same_r = pss_r * s_r - pss_i * s_i;
                 double same_i = pss_r * s_i + pss_i * s_r;
                 if (i != j) {
@@ -671,6 +678,10 @@ void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
     }
 }
 
+void oracle_rhs(const oracle_instance* inst, const double* in, double* out) { rhs_eval(inst, in, out, 0); }
+
+void oracle_rhs_tss(const oracle_instance* inst, const double* in, double* out) { rhs_eval(inst, in, out, 1); }
+
 /* ======================================================================== */
 /* RK (cl/heom/rk_weighted_add.cl:19-106 and the classical RK4 tableau)      */
 /* ======================================================================== */
@@ -678,7 +689,7 @@ void oracle_rhs(const oracle_instance* inst, const double* in, double* out)
 void oracle_rk_weighted_add(int64_t size, int n, double h, double* out, const double* y_n,
                             const double* c, const double* const* k)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (size > 65536)
     for (int64_t e = 0; e < size; ++e) {
         double v = 0.0;
         for (int l = 0; l < n; ++l)
@@ -741,7 +752,7 @@ int64_t oracle_propagate(const oracle_instance* inst, double h, int64_t nsteps,
 void oracle_hierarchy_norm(int states, int64_t matrices, const double* sigma, double threshold,
                            double* norms, uint8_t* mask)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (matrices * states * states > 16384)
     for (int64_t id = 0; id < matrices; ++id) {
         const double* a = sigma + id * states * states * 2;
         double acc = 0.0;
@@ -756,7 +767,7 @@ void oracle_hierarchy_norm(int states, int64_t matrices, const double* sigma, do
 void oracle_mmult_left(int states, int64_t matrices, const double* mx, double* sigma)
 {
     const int N = states;
-#pragma omp parallel
+#pragma omp parallel if (matrices * N * N > 16384)
     {
         double* tmp = (double*)malloc(sizeof(double) * (size_t)(N * N * 2));
 #pragma omp for schedule(static)
@@ -832,7 +843,7 @@ void oracle_rhs_adjoint(const oracle_instance* inst, const double* mu, double* o
     const double HB = ORACLE_HBAR;
     const double* H = inst->hamiltonian;
 
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel for schedule(dynamic, 64) if (M * N * N > 16384)
     for (int64_t id = 0; id < M; ++id) {
         const double* m = mu + id * N * N * 2;
         double* o = out + id * N * N * 2;
@@ -878,7 +889,7 @@ void oracle_rhs_adjoint(const oracle_instance* inst, const double* mu, double* o
         }
     }
 
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (N * N > 64)
     for (int e = 0; e < N * N; ++e) {
         const int i = e / N, j = e % N;
         for (int64_t id = 0; id < M; ++id) {
@@ -954,7 +965,7 @@ void oracle_adjoint_contract(const oracle_instance* inst, const double* mu, cons
     const int64_t M = inst->matrices;
     const double HB = ORACLE_HBAR;
 
-#pragma omp parallel
+#pragma omp parallel if (M * N * N > 16384)
     {
         double* lh = calloc((size_t)(2 * N * N + 4 * B + 2 * W), sizeof(double));
         double* ls = lh + 2 * N * N;
@@ -1069,7 +1080,7 @@ void oracle_adjoint_contract(const oracle_instance* inst, const double* mu, cons
 
 static void axpy_into(int64_t size, double* out, const double* x, double a, const double* y)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (size > 65536)
     for (int64_t e = 0; e < size; ++e)
         out[e] = x[e] + a * y[e];
 }
@@ -1272,3 +1283,78 @@ int64_t oracle_propagate_timedep(const oracle_instance* inst, double h, int64_t 
     }
     return rows;
 }
+
+/* ======================================================================== */
+/* thermal state search (include/heom/thermal_state_search_solver.hpp:267-299) */
+/* ======================================================================== */
+
+/* sigma <- sigma mx for every ADO (cl/heom/hierarchy_arithmetic_mmult_right.cl:15-50) */
+void oracle_mmult_right(int states, int64_t matrices, const double* mx, double* sigma)
+{
+    const int N = states;
+#pragma omp parallel if (matrices * N * N > 16384)
+    {
+        double* tmp = (double*)malloc(sizeof(double) * (size_t)(N * N * 2));
+#pragma omp for schedule(static)
+        for (int64_t id = 0; id < matrices; ++id) {
+            double* a = sigma + id * N * N * 2;
+            for (int i = 0; i < N; ++i) {
+                for (int j = 0; j < N; ++j) {
+                    double vr = 0.0, vi = 0.0;
+                    for (int k = 0; k < N; ++k) {
+                        const double ar = a[(i * N + k) * 2], ai = a[(i * N + k) * 2 + 1];
+                        const double mr = mx[(k * N + j) * 2], mi = mx[(k * N + j) * 2 + 1];
+                        vr = vr + (ar * mr - ai * mi);
+                        vi = vi + (ar * mi + ai * mr);
+                    }
+                    tmp[(i * N + j) * 2] = vr;
+                    tmp[(i * N + j) * 2 + 1] = vi;
+                }
+            }
+            memcpy(a, tmp, sizeof(double) * (size_t)(N * N * 2));
+        }
+        free(tmp);
+    }
+}
+
+/* One iteration of the search, in the reference's order: k = f_TSS(y) with the
+ * pseudo-Hamiltonian in inst->hamiltonian (formulas (17)-(20)); k <- C k C^T;
+ * y_u = k_u / (i (E_i - E_j) / hbar + eps - S_u) elementwise
+ * (cl/heom/thermal_state_search.cl:36-48, S_u = pf_same_sum); y <- C^T y C.
+ * c [N*N*2], eigenvalues [N*2], work [M*N*N*2]. */
+void oracle_tss_step(const oracle_instance* inst, const double* c, const double* eigenvalues, double eps_r,
+                     double eps_i, double* y, double* work)
+{
+    const int N = inst->states;
+    const int64_t M = inst->matrices;
+    double* ct = (double*)malloc(sizeof(double) * (size_t)(N * N * 2));
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            ct[(i * N + j) * 2] = c[(j * N + i) * 2];
+            ct[(i * N + j) * 2 + 1] = c[(j * N + i) * 2 + 1];
+        }
+    oracle_rhs_tss(inst, y, work);
+    oracle_mmult_left(N, M, c, work);
+    oracle_mmult_right(N, M, ct, work);
+#pragma omp parallel for schedule(static) if (M * N * N > 16384)
+    for (int64_t id = 0; id < M; ++id) {
+        const double pr = inst->pf_same_sum[2 * id], pi = inst->pf_same_sum[2 * id + 1];
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                /* (i / hbar) (E_i - E_j) + (eps - S_u) */
+                const double der = eigenvalues[2 * i] - eigenvalues[2 * j];
+                const double dei = eigenvalues[2 * i + 1] - eigenvalues[2 * j + 1];
+                const double dr = -dei / ORACLE_HBAR + (eps_r - pr);
+                const double di = der / ORACLE_HBAR + (eps_i - pi);
+                const int64_t e = (id * N * N + i * N + j) * 2;
+                const double ar = work[e], ai = work[e + 1];
+                const double den = dr * dr + di * di;
+                y[e] = (ar * dr + ai * di) / den;
+                y[e + 1] = (ai * dr - ar * di) / den;
+            }
+    }
+    oracle_mmult_left(N, M, ct, y);
+    oracle_mmult_right(N, M, c, y);
+    free(ct);
+}
+

@@ -69,6 +69,11 @@ typedef struct oracle_instance {
 
 /* out = f(in): pure evaluation (heom_ode.cl:430-434, neither ACCUMULATE nor SUBDIAGONAL) */
 void oracle_rhs(const oracle_instance* inst, const double* in, double* out);
+/* out = f(in) of the thermal state search (heom_ode.cl with HEOM_TSS_EQUATION,
+ * :116-121 and :130-136): the commutator becomes the elementwise product
+ * sigma_ij H_ij with the pseudo-Hamiltonian in inst->hamiltonian, and the S_n sigma
+ * term is dropped */
+void oracle_rhs_tss(const oracle_instance* inst, const double* in, double* out);
 
 /* out = y_n + h * sum_{l < n, c_l != 0} c_l k_l (cl/heom/rk_weighted_add.cl:19-106) */
 void oracle_rk_weighted_add(int64_t size, int n, double h, double* out, const double* y_n,
@@ -116,6 +121,11 @@ int oracle_adjoint_gradient(const oracle_instance* inst, double h, int64_t nstep
                             double* observed, double* g_h, double* g_pss, double* g_same, double* g_circ,
                             double* g_cbk, double* g_state);
 /* number of OpenMP threads the oracle runs on */
+/* sigma <- sigma mx for every ADO (hierarchy_arithmetic_mmult_right.cl:15-50) */
+void oracle_mmult_right(int states, int64_t matrices, const double* mx, double* sigma);
+/* one thermal state search iteration (thermal_state_search_solver.hpp:267-299) */
+void oracle_tss_step(const oracle_instance* inst, const double* c, const double* eigenvalues, double eps_r,
+                     double eps_i, double* y, double* work);
 int oracle_num_threads(void);
 
 #ifdef __cplusplus

@@ -15,7 +15,15 @@ pieces the reference keeps in C++:
 * the instance assembly (``src/heom/instance.cpp:25-69, 483-495``);
 * the dipole matrices of the linear-absorption pseudo pathway
   (``include/heom/dipole_matrix.hpp:30-74``, ``src/heom/dipole_matrix.cpp:182-208``,
-  ``src/app_linear_absorption.cpp:88-163``).
+  ``src/app_linear_absorption.cpp:88-163``);
+* the thermal state search set-up and loop
+  (``include/heom/thermal_state_search_solver.hpp:124-325``,
+  ``src/app_thermal_state_search.cpp:84-128``), with numpy's eigh in place of the
+  un-vendored Eigen ComplexEigenSolver (pinned against the dense generator's null vector
+  in tests/test_oracle_tss.py);
+* the binary instance file (``src/heom/instance.cpp:281-377``), read and written with
+  numpy independently of the product's C++ (no golden file exists in the reference; the
+  layout is pinned by its field list and the size formula).
 """
 from __future__ import annotations
 
@@ -105,6 +113,12 @@ def lib():
         L.oracle_propagate_timedep.restype = ctypes.c_int64
         L.oracle_propagate_timedep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
                                                f64p, f64p, f64p, f64p]
+        L.oracle_rhs_tss.restype = None
+        L.oracle_rhs_tss.argtypes = [ctypes.c_void_p, f64p, f64p]
+        L.oracle_mmult_right.restype = None
+        L.oracle_mmult_right.argtypes = [ctypes.c_int, ctypes.c_int64, f64p, f64p]
+        L.oracle_tss_step.restype = None
+        L.oracle_tss_step.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_double, ctypes.c_double, f64p, f64p]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_num_threads.argtypes = []
         _lib_handle = L
@@ -241,6 +255,7 @@ _REQUIRED_TASK = {
     "population_dynamics": ["population_dynamics.rho_init"],
     "linear_absorption": ["dipole.directions", "dipole.strengths", "dipole.centers",
                           "dipole.tensor_prefactors", "dipole.tensor_components"],
+    "thermal_state_search": ["thermal_state_search.delta", "thermal_state_search.max_steps"],
 }
 
 
@@ -269,6 +284,8 @@ class Config:
     baths_temperature: float
     rho_init: np.ndarray | None = None
     dipole: dict = field(default_factory=dict)
+    tss_delta: float = 0.0
+    tss_max_steps: int = 0
 
 
 def parse_config(path: str, task: str | None = None) -> Config:
@@ -316,6 +333,10 @@ def parse_config(path: str, task: str | None = None) -> Config:
         raise ConfigError(f"faulty value: {e}")
     if t == "population_dynamics":
         cfg.rho_init = _as_complex_matrix(lit["population_dynamics.rho_init"], "population_dynamics.rho_init")
+    elif t == "thermal_state_search":
+        # thermal_state_search_config.cpp:19-23
+        cfg.tss_delta = float(lit["thermal_state_search.delta"])
+        cfg.tss_max_steps = int(lit["thermal_state_search.max_steps"])
     else:
         cfg.dipole = {
             "directions": np.array(lit["dipole.directions"], dtype=np.float64),
@@ -382,6 +403,9 @@ def check_config(c: Config) -> None:
     if c.task == "population_dynamics":
         if c.rho_init.shape != (c.sites, c.sites):
             raise ConfigError("population_dynamics.rho_init must be a system.sites by system.sites matrix")
+    elif c.task == "thermal_state_search":
+        if c.step_size != 1.0:  # thermal_state_search_config.cpp:37-38
+            raise ConfigError("solver.step_size_ must equal 1.0")
     else:
         d = c.dipole
         if not (len(d["directions"]) == len(d["strengths"]) == len(d["centers"])):
@@ -544,6 +568,10 @@ class Instance:
             self._c = c
         return ctypes.byref(self._c)
 
+    def hamiltonian_matrix(self) -> np.ndarray:
+        h = self.hamiltonian.reshape(self.states, self.states, 2)
+        return h[..., 0] + 1j * h[..., 1]
+
     def set_hamiltonian(self, H: np.ndarray) -> None:
         self.hamiltonian[:] = np.stack([H.real, H.imag], -1).reshape(-1)
 
@@ -598,6 +626,27 @@ class Instance:
         return t[..., 0] + 1j * t[..., 1]
 
     # -- adjoint -----------------------------------------------------------------------
+    def mmult_right(self, mx: np.ndarray, state: np.ndarray) -> None:
+        m = np.ascontiguousarray(np.stack([mx.real, mx.imag], -1).reshape(-1), dtype=np.float64)
+        lib().oracle_mmult_right(self.states, self.matrices, m, state)
+
+    def rhs_tss(self, state: np.ndarray) -> np.ndarray:
+        """f of the thermal state search (HEOM_TSS_EQUATION) with this instance's
+        Hamiltonian as the pseudo-Hamiltonian."""
+        out = np.zeros_like(state)
+        lib().oracle_rhs_tss(self.cstruct(), np.ascontiguousarray(state), out)
+        return out
+
+    def tss_step(self, y: np.ndarray, setup: dict, n: int = 1) -> None:
+        """n iterations of the thermal state search in place (the instance's Hamiltonian
+        must be the pseudo-Hamiltonian, thermal_state_search_solver.hpp:199-209)."""
+        c = np.ascontiguousarray(np.stack([setup["c"].real, setup["c"].imag], -1).reshape(-1))
+        ev = np.ascontiguousarray(np.stack([setup["eigenvalues"].real, setup["eigenvalues"].imag], -1).reshape(-1))
+        eps = complex(setup["epsilon"])
+        work = np.zeros_like(y)
+        for _ in range(n):
+            lib().oracle_tss_step(self.cstruct(), c, ev, eps.real, eps.imag, y, work)
+
     def rhs_adjoint(self, mu: np.ndarray) -> np.ndarray:
         """L^dag mu, the transpose of rhs() (heom_oracle.c: oracle_rhs_adjoint)."""
         out = np.empty_like(mu)
@@ -693,6 +742,62 @@ def field_stage_hamiltonians(H0: np.ndarray, d_minus: np.ndarray, d_plus: np.nda
     times = t0 + h * (np.arange(nsteps)[:, None] + c[None, :])
     E = np.vectorize(field, otypes=[np.complex128])(times)
     return H0[None, None] - E[..., None, None] * d_minus[None, None] - np.conj(E)[..., None, None] * d_plus[None, None]
+
+
+# --------------------------------------------------------------------------------------
+# thermal state search (include/heom/thermal_state_search_solver.hpp:124-300,
+# src/app_thermal_state_search.cpp:84-128)
+# --------------------------------------------------------------------------------------
+
+TSS_GROUND_STATE_ENERGY = 100000.0  # include/heom/constants.hpp:27
+TSS_EPSILON = 1.0e14                # thermal_state_search_solver.hpp:201-206 ("Version C")
+
+
+def thermal_state_setup(H: np.ndarray, temperature: float, mode: str = "identity") -> dict:
+    """thermal_state_search_solver::initialize (:124-209) for a real symmetric state
+    Hamiltonian H (J): eigenvectors V (numpy's eigh in place of the un-vendored Eigen
+    ComplexEigenSolver), C = V^T, the eigenvalues, the Boltzmann top
+    sigma_0 = C^T diag(exp(-beta E) / Z) C and the constant epsilon of the
+    pseudo-Hamiltonian.  With a ground state its energy is raised to keep it empty."""
+    H = np.array(H, dtype=np.complex128)
+    if mode == "with_ground_state":
+        H[0, 0] = TSS_GROUND_STATE_ENERGY
+    if np.any(H.imag != 0):
+        raise ValueError("the thermal state search needs a real symmetric Hamiltonian")
+    E, V = np.linalg.eigh(H.real)
+    beta = 1.0 / (KB * temperature)
+    p = np.exp(-(E - E.min()) * beta)
+    p /= p.sum()
+    C = V.T.astype(np.complex128)
+    top = C.T @ np.diag(p) @ C
+    N = H.shape[0]
+    return {"c": C, "eigenvalues": E.astype(np.complex128), "top": top, "epsilon": complex(TSS_EPSILON),
+            "pseudo_hamiltonian": np.full((N, N), TSS_EPSILON, dtype=np.complex128)}
+
+
+def thermal_state_search(cfg: Config, graph: Graph | None = None, max_iterations: int | None = None):
+    """app_thermal_state_search main loop (src/app_thermal_state_search.cpp:84-128): at
+    least solver.steps / observe_steps iterations of observe_steps search steps, then
+    more until the 1-norm of the top changes by less than delta across an iteration or
+    max_steps / observe_steps iterations are done.  Returns (tops [iterations + 1, N, N],
+    final state, iterations)."""
+    inst = instance_from_config(cfg, "identity", graph)
+    setup = thermal_state_setup(inst.hamiltonian_matrix(), cfg.baths_temperature)
+    inst.set_hamiltonian(setup["pseudo_hamiltonian"])
+    y = inst.state_with_top(setup["top"])
+    min_it = cfg.steps // cfg.observe_steps
+    max_it = max(min_it, cfg.tss_max_steps // cfg.observe_steps)
+    if max_iterations is not None:
+        max_it = min(max_it, max_iterations)
+    tops = [inst.top(y)]
+    it = 0
+    while it < max_it:
+        inst.tss_step(y, setup, cfg.observe_steps)
+        tops.append(inst.top(y))
+        it += 1
+        if it > min_it and abs(np.abs(tops[-2]).sum() - np.abs(tops[-1]).sum()) < cfg.tss_delta:
+            break
+    return np.array(tops), y, it
 
 
 # --------------------------------------------------------------------------------------

@@ -46,7 +46,7 @@ def test_abi_version_and_error_path_without_gpu():
 
 
 def test_apps_print_usage():
-    for app in ("heom_population_dynamics", "heom_linear_absorption"):
+    for app in ("heom_population_dynamics", "heom_linear_absorption", "heom_thermal_state_search"):
         exe = os.path.join(ROOT, "dm-heom_amd", "bin", app)
         assert os.path.exists(exe)
         r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)

@@ -1,11 +1,11 @@
 """The product's applications end to end on the GPU against the oracle.
 
-heom_population_dynamics and heom_linear_absorption (dm-heom_amd/bin, the counterparts
-of dm-heom/src/app_population_dynamics.cpp and app_linear_absorption.cpp) read a
-reference config file, build the instance with the C++ host library, propagate with the
-HIP stage kernels and write observation files.  Their numbers are compared with the
-oracle's propagation of the same config: populations and absorption traces within
-1e-10 relative to the largest value.
+heom_population_dynamics, heom_linear_absorption and heom_thermal_state_search
+(dm-heom_amd/bin, the counterparts of dm-heom/src/app_population_dynamics.cpp,
+app_linear_absorption.cpp and app_thermal_state_search.cpp) read a reference config
+file, build the instance with the C++ host library, run the HIP kernels and write
+observation files.  Their numbers are compared with the oracle's run of the same config:
+populations and absorption traces within 1e-10 relative to the largest value.
 """
 import os
 import re
@@ -93,3 +93,25 @@ def test_linear_absorption_app_fmo(oracle_mod, tmp_path, rotating):
     err = np.max(np.abs(val - ref)) / np.max(np.abs(ref))
     print(f"FMO absorption trace (rotating={rotating}): max rel deviation {err:.3e}", flush=True)
     assert err < 1e-10
+
+
+def test_thermal_state_search_app_fmo(oracle_mod, tmp_path):
+    """The FMO thermal-state example as shipped (500 steps at least, delta 1e-8): one row
+    per iteration, the same iteration count and converged populations as the oracle's
+    search loop."""
+    cfg_path = write_cfg(os.path.join(FMO_DIR, "fmo_thermal_state_77K.cfg"), tmp_path / "tss.cfg",
+                         observations="{(matrix_diagonal, tss_diag.dat), (matrix_trace_id, tss_trace.dat)}")
+    r = run_app("heom_thermal_state_search", str(cfg_path), tmp_path)
+    assert "converged" in r.stdout and "not converged" not in r.stdout
+    header, got = read_table(tmp_path / "tss_diag.dat")
+    assert header == ["time"] + [f"site_{i}" for i in range(7)]
+    cfg = O.parse_config(str(cfg_path))
+    tops, _, it = O.thermal_state_search(cfg)
+    pops = np.real(np.diagonal(tops, axis1=1, axis2=2))
+    assert got.shape == (it + 1, 8)
+    assert np.allclose(got[:, 0], np.arange(it + 1) * cfg.observe_steps, rtol=0, atol=0)
+    err = np.max(np.abs(got[:, 1:] - pops))
+    print(f"FMO thermal state: {it} iterations, max population deviation {err:.1e}", flush=True)
+    assert err < 1e-10
+    _, tr = read_table(tmp_path / "tss_trace.dat")
+    assert np.max(np.abs(tr[:, 1] - 1.0)) < 1e-9

@@ -1,0 +1,118 @@
+"""The thermal state search on the GPU (heom_create_tss / heom_tss_step) against the
+oracle's oracle_tss_step on the same inputs: f_TSS (heom_rhs on a search handle), single
+and repeated iterations for the fused transform (N <= 8) and the four-pass one (N = 24),
+a filtered hierarchy, the full search of the FMO thermal-state config to convergence,
+and the entry points a search handle refuses."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import FMO_DIR, instance, max_abs_rel, random_state
+
+pytestmark = pytest.mark.gpu
+
+heom_amd = pytest.importorskip("heom_amd")
+
+TSS_CFG = FMO_DIR + "/fmo_thermal_state_77K.cfg"
+
+
+def search_case(name=None, depth=None):
+    if name is None:
+        cfg = O.parse_config(TSS_CFG)
+        inst = O.instance_from_config(cfg)
+    else:
+        cfg, inst = instance(name, depth=depth)
+    s = O.thermal_state_setup(inst.hamiltonian_matrix(), cfg.baths_temperature)
+    inst.set_hamiltonian(s["pseudo_hamiltonian"])
+    return cfg, inst, s
+
+
+CASES = [(None, None), ("excitonic8_d6", 3), ("excitonic7_d4", None), ("excitonic24_d4", 1)]
+
+
+@pytest.mark.parametrize("name,depth", CASES)
+def test_rhs_tss_matches_oracle(oracle_mod, name, depth):
+    _, inst, s = search_case(name, depth)
+    y = random_state(inst, 1)
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+        h.set_state(y)
+        got = h.rhs()
+    err = max_abs_rel(got, inst.rhs_tss(y))
+    print(f"{name or 'fmo_thermal_state'}: f_TSS max rel deviation {err:.1e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("name,depth", CASES)
+def test_tss_steps_match_oracle(oracle_mod, name, depth):
+    _, inst, s = search_case(name, depth)
+    y = random_state(inst, 2)
+    ref = y.copy()
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+        h.set_state(y)
+        h.tss_step(1)
+        one = h.get_state()
+        h.tss_step(9)
+        ten = h.get_state()
+    inst.tss_step(ref, s, 1)
+    e1 = max_abs_rel(one, ref)
+    inst.tss_step(ref, s, 9)
+    e10 = max_abs_rel(ten, ref)
+    print(f"{name or 'fmo_thermal_state'} ({inst.matrices} ADOs, N={inst.states}): 1 step {e1:.1e}, "
+          f"10 steps {e10:.1e}", flush=True)
+    assert e1 < 1e-10 and e10 < 1e-10
+
+
+def test_masked_search_matches_oracle(oracle_mod):
+    _, inst, s = search_case("excitonic8_d6", 3)
+    rng = np.random.default_rng(3)
+    mask = np.where(rng.random(inst.matrices) < 0.3, O.MASK_FALSE, O.MASK_TRUE).astype(np.uint8)
+    mask[0] = O.MASK_PROTECTED
+    y = random_state(inst, 4)
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+        h.set_mask(mask)
+        h.set_state(y)
+        h.tss_step(5)
+        got = h.get_state()
+    inst.mask[:] = mask
+    inst.tss_step(y, s, 5)
+    keep = np.repeat(mask != O.MASK_FALSE, inst.states * inst.states * 2)
+    assert max_abs_rel(got[keep], y[keep]) < 1e-10
+
+
+def test_fmo_search_to_convergence(oracle_mod):
+    """The reference loop (app_thermal_state_search.cpp:84-128) on the device, against the
+    oracle's: same iteration count, same converged top."""
+    cfg = O.parse_config(TSS_CFG)
+    tops_ref, y_ref, it_ref = O.thermal_state_search(cfg)
+    host = heom_amd.HostInstance(TSS_CFG, task=heom_amd.TASK_THERMAL_STATE_SEARCH)
+    with host.create_tss_handle() as h:
+        last = h.get_top()
+        assert np.max(np.abs(last - tops_ref[0])) < 1e-14
+        it = 0
+        min_it = cfg.steps // cfg.observe_steps
+        while it < cfg.tss_max_steps // cfg.observe_steps:
+            h.tss_step(cfg.observe_steps)
+            it += 1
+            top = h.get_top()
+            done = it > min_it and abs(np.abs(last).sum() - np.abs(top).sum()) < cfg.tss_delta
+            last = top
+            if done:
+                break
+        y = h.get_state()
+    print(f"FMO thermal state: {it} iterations (oracle {it_ref}), top deviation "
+          f"{np.max(np.abs(top - tops_ref[-1])):.1e}", flush=True)
+    assert it == it_ref
+    assert np.max(np.abs(top - tops_ref[-1])) < 1e-10
+    assert max_abs_rel(y, y_ref) < 1e-8
+
+
+def test_search_handle_refuses_propagation(oracle_mod):
+    _, inst, s = search_case("excitonic8_d6", 2)
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+        for call in (lambda: h.step_rk4(1e-15, 1), lambda: h.propagate(1e-15, 2, 1),
+                     lambda: h.set_hamiltonian(np.eye(inst.states)), lambda: h.compact()):
+            with pytest.raises(heom_amd.HeomError, match="thermal-state-search"):
+                call()
+    with heom_amd.Handle.from_instance(inst) as h:
+        with pytest.raises(heom_amd.HeomError, match="not a thermal-state-search handle"):
+            h.tss_step(1)
