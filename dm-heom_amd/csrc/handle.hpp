@@ -98,6 +98,7 @@ struct TssState {
     DevBuf<double2> c, ct;  // C and C^T [nel]
     DevBuf<double2> w;      // (i/hbar)(E_i - E_j) [nel]
     DevBuf<double2> epsm;   // eps - S_u per slot [Mp]
+    bool real_c = false;    // C has no imaginary parts (real symmetric H)
 };
 
 }  // namespace heom_rt

@@ -56,7 +56,17 @@ struct TssArgs {
 // C k needs all rows of k (LDS); (C k) C^T and the division act on row i alone; C^T B
 // needs all rows of B (LDS again, after every wavefront has finished reading k); the
 // final right product acts on row i.
-template <int N>
+// c * x (+ acc) with C's entries real when REALC (the eigenvectors of a real symmetric
+// H): two multiplies per product instead of four
+template <bool REALC>
+__device__ __forceinline__ double2 cmac(double2 c, double2 x, double2 acc)
+{
+    if (REALC)
+        return make_double2(fma(c.x, x.x, acc.x), fma(c.x, x.y, acc.y));
+    return cfma(c, x, acc);
+}
+
+template <int N, bool REALC>
 __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
 {
     __shared__ double2 t[N * N * kTile];
@@ -76,7 +86,7 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         const double2 cim = a.c[i * N + m];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            r[j] = cfma(cim, t[(m * N + j) * kTile + lane], r[j]);
+            r[j] = cmac<REALC>(cim, t[(m * N + j) * kTile + lane], r[j]);
     }
     const double2 em = a.epsm[q];
     const bool pad = a.q2u[q] < 0;
@@ -85,7 +95,7 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         double2 v = make_double2(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < N; ++m)
-            v = cfma(r[m], a.ct[m * N + j], v);
+            v = cmac<REALC>(a.ct[m * N + j], r[m], v);
         const double2 wij = a.w[i * N + j];
         s[j] = cdiv(v, make_double2(wij.x + em.x, wij.y + em.y));
     }
@@ -102,14 +112,14 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         const double2 ctim = a.ct[i * N + m];
 #pragma unroll
         for (int j = 0; j < N; ++j)
-            r[j] = cfma(ctim, t[(m * N + j) * kTile + lane], r[j]);
+            r[j] = cmac<REALC>(ctim, t[(m * N + j) * kTile + lane], r[j]);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {  // row i of C^T B C
         double2 v = make_double2(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < N; ++m)
-            v = cfma(r[m], a.c[m * N + j], v);
+            v = cmac<REALC>(a.c[m * N + j], r[m], v);
         a.y[(int64_t)(i * N + j) * a.mp + q] = pad ? make_double2(0.0, 0.0) : v;
     }
 }
@@ -141,6 +151,15 @@ __global__ void tss_pass(const double2* __restrict__ in, double2* __restrict__ o
         v = cdiv(v, make_double2(w[e].x + em.x, w[e].y + em.y));
     }
     out[t] = q2u[q] < 0 ? make_double2(0.0, 0.0) : v;
+}
+
+template <int N>
+void launch_transform(heom_handle* h, const TssArgs& a, bool real_c)
+{
+    if (real_c)
+        hipLaunchKernelGGL((tss_transform<N, true>), dim3(h->ntiles), dim3(64 * N), 0, h->stream, a);
+    else
+        hipLaunchKernelGGL((tss_transform<N, false>), dim3(h->ntiles), dim3(64 * N), 0, h->stream, a);
 }
 
 void check_tss(heom_handle* h)
@@ -230,6 +249,9 @@ int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenv
             b.alloc(v.size());
             HIPC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
         };
+        T->real_c = true;
+        for (int e = 0; e < nel; ++e)
+            T->real_c = T->real_c && cv[e].y == 0.0;
         up(T->c, cv);
         up(T->ct, ctv);
         up(T->w, wv);
@@ -263,14 +285,14 @@ int heom_tss_step(heom_handle* h, int64_t n)
         for (int64_t s = 0; s < n; ++s) {
             launch_stage_buffers(h, 0, h->y.p, h->y.p, h->pbuf.p, h->ya.p, 0.0, false, false);
             switch (N) {
-            case 1: hipLaunchKernelGGL(tss_transform<1>, dim3(h->ntiles), dim3(64 * 1), 0, h->stream, a); break;
-            case 2: hipLaunchKernelGGL(tss_transform<2>, dim3(h->ntiles), dim3(64 * 2), 0, h->stream, a); break;
-            case 3: hipLaunchKernelGGL(tss_transform<3>, dim3(h->ntiles), dim3(64 * 3), 0, h->stream, a); break;
-            case 4: hipLaunchKernelGGL(tss_transform<4>, dim3(h->ntiles), dim3(64 * 4), 0, h->stream, a); break;
-            case 5: hipLaunchKernelGGL(tss_transform<5>, dim3(h->ntiles), dim3(64 * 5), 0, h->stream, a); break;
-            case 6: hipLaunchKernelGGL(tss_transform<6>, dim3(h->ntiles), dim3(64 * 6), 0, h->stream, a); break;
-            case 7: hipLaunchKernelGGL(tss_transform<7>, dim3(h->ntiles), dim3(64 * 7), 0, h->stream, a); break;
-            case 8: hipLaunchKernelGGL(tss_transform<8>, dim3(h->ntiles), dim3(64 * 8), 0, h->stream, a); break;
+            case 1: launch_transform<1>(h, a, T.real_c); break;
+            case 2: launch_transform<2>(h, a, T.real_c); break;
+            case 3: launch_transform<3>(h, a, T.real_c); break;
+            case 4: launch_transform<4>(h, a, T.real_c); break;
+            case 5: launch_transform<5>(h, a, T.real_c); break;
+            case 6: launch_transform<6>(h, a, T.real_c); break;
+            case 7: launch_transform<7>(h, a, T.real_c); break;
+            case 8: launch_transform<8>(h, a, T.real_c); break;
             default:
                 // C k -> yb, (. C^T) / D -> ya, C^T . -> yb, . C -> y, in the reference's order
                 hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, ya, yb, T.c.p, nullptr, nullptr, h->q2u.p, h->Mp, N, 1);

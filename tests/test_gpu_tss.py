@@ -116,3 +116,24 @@ def test_search_handle_refuses_propagation(oracle_mod):
     with heom_amd.Handle.from_instance(inst) as h:
         with pytest.raises(heom_amd.HeomError, match="not a thermal-state-search handle"):
             h.tss_step(1)
+
+
+@pytest.mark.parametrize("name,depth", [(None, None), ("excitonic24_d4", 1)])
+def test_complex_transform_matches_oracle(oracle_mod, name, depth):
+    """The transform for any complex C (the real-C kernel serves real symmetric H; the
+    general one is what the reference's kernels apply to any C and eigenvalues)."""
+    _, inst, s = search_case(name, depth)
+    N = inst.states
+    rng = np.random.default_rng(5)
+    q, _ = np.linalg.qr(rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N)))
+    cs = dict(s, c=q, eigenvalues=s["eigenvalues"] + 1j * rng.normal(size=N) * 1e-22)
+    y = random_state(inst, 6)
+    ref = y.copy()
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, cs) as h:
+        h.set_state(y)
+        h.tss_step(3)
+        got = h.get_state()
+    inst.tss_step(ref, cs, 3)
+    err = max_abs_rel(got, ref)
+    print(f"{name or 'fmo_thermal_state'}: complex C, 3 steps {err:.1e}", flush=True)
+    assert err < 1e-10
