@@ -130,6 +130,11 @@ struct heom_handle {
     int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
     int wide_ados = 0;       // N > 8: ADOs per workgroup of the wide kernel (0 = the most that fit)
 
+    // tuple-keyed readback (heom_get_ados): each instance's tuples in lexicographic
+    // order [M][W] with the uid of each, and the current slot of every uid (-1: removed)
+    std::vector<uint8_t> lex_tuples;
+    std::vector<int32_t> lex_uid, u2q_host;
+
     std::vector<int32_t> coupling;  // [N * BM]
     std::vector<double> cbk, pf_same, pf_circ;
 

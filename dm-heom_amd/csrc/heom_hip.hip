@@ -906,6 +906,9 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     std::vector<int32_t> q2u(Mp, -1), tinst(h->ntiles, count - 1);
     std::vector<int64_t> qtop(count);
     std::vector<std::vector<int32_t>> u2q_all(count);
+    h->lex_tuples.assign((size_t)h->M * W, 0);
+    h->lex_uid.assign((size_t)h->M, 0);
+    h->u2q_host.assign((size_t)h->M, -1);
     for (int k = 0; k < count; ++k) {
         const int64_t M = d[k].matrices, qb = h->inst_q[k];
         const int32_t* T = d[k].ado_tuples;
@@ -927,6 +930,12 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
         for (int64_t q = 0; q < M; ++q) {
             u2q[order[q]] = (int32_t)q;
             q2u[qb + q] = (int32_t)(h->inst_u[k] + order[q]);
+            // the sorted tuples of heom_get_ados: lexicographic position -> uid
+            const int64_t lx = h->inst_u[k] + q;
+            for (int m = 0; m < W; ++m)
+                h->lex_tuples[(size_t)lx * W + m] = (uint8_t)T[(int64_t)order[q] * W + m];
+            h->lex_uid[(size_t)lx] = (int32_t)(h->inst_u[k] + order[q]);
+            h->u2q_host[(size_t)(h->inst_u[k] + order[q])] = (int32_t)(qb + q);
         }
         qtop[k] = qb + u2q[0];
         for (int64_t t = qb / kTile; t < (qb + M + kTile - 1) / kTile; ++t)
@@ -1230,6 +1239,87 @@ int heom_get_top(heom_handle* h, double* top)
         HIPC(hipMemcpyAsync(top, h->small.p, (size_t)h->count * h->nel * sizeof(double2), hipMemcpyDeviceToHost,
                             h->stream));
         HIPC(hipStreamSynchronize(h->stream));
+    });
+}
+
+int heom_get_ados(heom_handle* h, int32_t instance, const int32_t* tuples, int64_t n, double* out)
+{
+    return guard([&] {
+        need(h);
+        if (instance < 0 || instance >= h->count)
+            raise(HEOM_E_INVALID, "instance out of range");
+        if (n < 0 || (n > 0 && (!tuples || !out)))
+            raise(HEOM_E_INVALID, "n must be >= 0, tuples and out non-NULL");
+        const int W = h->W, nel = h->nel;
+        const int64_t lo = h->inst_u[instance], hi = lo + h->inst_M[instance];
+        std::vector<uint8_t> key((size_t)W);
+        std::vector<int64_t> slot((size_t)n), uid((size_t)n);
+        for (int64_t a = 0; a < n; ++a) {
+            for (int m = 0; m < W; ++m) {
+                const int32_t v = tuples[a * W + m];
+                if (v < 0 || v > kMaxTupleValue)
+                    raise(HEOM_E_INVALID, "tuple " + std::to_string(a) + " is not an ADO of the instance");
+                key[(size_t)m] = (uint8_t)v;
+            }
+            // binary search over the instance's sorted tuples
+            int64_t l = lo, r = hi;
+            while (l < r) {
+                const int64_t mid = l + (r - l) / 2;
+                const uint8_t* t = &h->lex_tuples[(size_t)mid * W];
+                if (std::lexicographical_compare(t, t + W, key.begin(), key.end()))
+                    l = mid + 1;
+                else
+                    r = mid;
+            }
+            if (l == hi || !std::equal(key.begin(), key.end(), &h->lex_tuples[(size_t)l * W]))
+                raise(HEOM_E_INVALID, "tuple " + std::to_string(a) + " is not an ADO of the instance");
+            uid[(size_t)a] = h->lex_uid[(size_t)l];
+            slot[(size_t)a] = h->u2q_host[(size_t)uid[(size_t)a]];  // -1: removed by heom_compact
+        }
+        std::fill(out, out + (size_t)n * nel * 2, 0.0);
+        std::vector<int64_t> live, at;
+        for (int64_t a = 0; a < n; ++a)
+            if (slot[(size_t)a] >= 0) {
+                live.push_back(slot[(size_t)a]);
+                at.push_back(a);
+            }
+        constexpr int64_t kChunk = 1 << 20;  // ADOs per gather launch (nel * chunk < 2^31)
+        DevBuf<int64_t> qs;
+        DevBuf<double2> buf;
+        std::vector<double2> host;
+        for (int64_t c0 = 0; c0 < (int64_t)live.size(); c0 += kChunk) {
+            const int cnt = (int)std::min<int64_t>(kChunk, (int64_t)live.size() - c0);
+            if (qs.n < (size_t)cnt) {
+                qs.alloc((size_t)cnt);
+                buf.alloc((size_t)cnt * nel);
+            }
+            HIPC(hipMemcpyAsync(qs.p, live.data() + c0, (size_t)cnt * sizeof(int64_t), hipMemcpyHostToDevice,
+                                h->stream));
+            const dim3 grid(blocks_for((int64_t)cnt * nel, 256)), block(256);
+            if (h->fp32())
+                hipLaunchKernelGGL(gather_ados<float2>, grid, block, 0, h->stream, h->y.as<float2>(), buf.p, qs.p, cnt,
+                                   h->Mp, nel);
+            else
+                hipLaunchKernelGGL(gather_ados<double2>, grid, block, 0, h->stream, h->y.as<double2>(), buf.p, qs.p,
+                                   cnt, h->Mp, nel);
+            HIPC(hipGetLastError());
+            host.resize((size_t)cnt * nel);
+            HIPC(hipMemcpyAsync(host.data(), buf.p, host.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            for (int b = 0; b < cnt; ++b)
+                std::memcpy(out + (size_t)at[(size_t)(c0 + b)] * nel * 2, &host[(size_t)b * nel],
+                            (size_t)nel * sizeof(double2));
+        }
+        if (h->fp32() && n > 0) {
+            // tier-0/1 ADOs: their FP64 copies, as heom_get_state returns them
+            std::vector<double2> sh((size_t)h->S * nel);
+            HIPC(hipMemcpyAsync(sh.data(), h->y64.p, sh.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+            for (int sp = 0; sp < h->S; ++sp)
+                for (int64_t a = 0; a < n; ++a)
+                    if (uid[(size_t)a] == h->sp_uid[(size_t)sp])
+                        std::memcpy(out + (size_t)a * nel * 2, &sh[(size_t)sp * nel], (size_t)nel * sizeof(double2));
+        }
     });
 }
 
@@ -1644,6 +1734,10 @@ int heom_compact(heom_handle* h, int64_t* active)
         h->active = kept;
         h->inst_q = new_q;
         h->inst_n = new_n;
+        std::fill(h->u2q_host.begin(), h->u2q_host.end(), -1);
+        for (int64_t qn = 0; qn < mp; ++qn)
+            if (q2u_n[qn] >= 0)
+                h->u2q_host[(size_t)q2u_n[qn]] = (int32_t)qn;
         refold_tables(h);
         drop_graph(h);
         adjoint_invalidate(h);

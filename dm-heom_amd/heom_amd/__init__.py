@@ -39,7 +39,7 @@ EXPORTED = [
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
-    "heom_propagate_timedep", "heom_create_tss", "heom_tss_step",
+    "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -135,6 +135,7 @@ def load():
             "heom_propagate_timedep": [vp, dbl, i64, i64, vp, vp],
             "heom_create_tss": [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
             "heom_tss_step": [vp, i64],
+            "heom_get_ados": [vp, ctypes.c_int32, vp, i64, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -350,6 +351,16 @@ class Handle:
         out = np.empty(self.state_size)
         _check(load().heom_get_state(self._h, out.ctypes.data))
         return out
+
+    def get_ados(self, tuples, instance: int = 0) -> np.ndarray:
+        """The ADOs with the given tuples ([n, W] ints) of one instance: complex [n, N, N]."""
+        t = np.ascontiguousarray(tuples, dtype=np.int32)
+        if t.ndim != 2 or t.shape[1] != self.width:
+            raise ValueError("tuples must be [n, W]")
+        out = np.zeros(t.shape[0] * self.states * self.states * 2)
+        _check(load().heom_get_ados(self._h, int(instance), t.ctypes.data, t.shape[0], out.ctypes.data))
+        o = out.reshape(t.shape[0], self.states, self.states, 2)
+        return o[..., 0] + 1j * o[..., 1]
 
     def get_top(self) -> np.ndarray:
         """The top ADO, (N, N); for a batch (count, N, N)."""

@@ -130,6 +130,12 @@ int heom_clear_state(heom_handle* handle);
 int heom_get_state(heom_handle* handle, double* state /* [M*N*N*2] */);
 /* solver_base::hierarchy_top (solver_base.hpp:219-222) */
 int heom_get_top(heom_handle* handle, double* top /* [count][N*N*2] */);
+/* Tuple-keyed readback, so that no ADO order is part of the contract: the ADOs of
+ * instance `instance` whose tuples (n_0 .. n_{W-1}, the reference's ado_tuples rows,
+ * include/heom/instance.hpp:125-126) are given, out [n][N*N*2].  A tuple that is not an
+ * ADO of the instance is an error; ADOs removed by heom_compact read as zero. */
+int heom_get_ados(heom_handle* handle, int32_t instance, const int32_t* tuples /* [n][W] */, int64_t n,
+                  double* out);
 /* solver_base::update_hierarchy_norm + hierarchy_norm (solver_base.hpp:224-228,
  * 270-273; kernel cl/heom/hierarchy_norm.cl:15-45): norms[n] = sum_ij |sigma_n,ij|^2 in
  * uid order; ADOs with norm - threshold <= 0 that are not PROTECTED are masked
