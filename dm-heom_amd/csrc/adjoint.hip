@@ -55,12 +55,31 @@ struct ContractArgs {
     int64_t mp;
 };
 
+// v moved across lanes by one DPP pattern (both 32-bit halves); lanes of rows outside
+// ROWS read 0, the identity of the sum
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes, wave-uniform result.  DPP lane moves instead of LDS-crossbar
+// shuffles: pairs and quads (quad_perm), the 8-lane halves and the 16-lane rows of each
+// row (row_half_mirror, row_mirror), then rows 0 + 1 into row 1 and rows 2 + 3 into row
+// 3 (row_bcast:15), and rows 0-1 into row 3 (row_bcast:31); lane 63 holds the total.
 __device__ __forceinline__ double wave_sum(double x)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        x += __shfl_xor(x, off);
-    return x;
+    x += dpp_move<0xb1, 0xf>(x);   // quad_perm [1, 0, 3, 2]
+    x += dpp_move<0x4e, 0xf>(x);   // quad_perm [2, 3, 0, 1]
+    x += dpp_move<0x141, 0xf>(x);  // row_half_mirror
+    x += dpp_move<0x140, 0xf>(x);  // row_mirror
+    x += dpp_move<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_move<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+    return __hiloint2double(hi, lo);
 }
 
 // conj(a) * b
@@ -69,11 +88,14 @@ __device__ __forceinline__ double2 cdot(double2 a, double2 b)
     return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
 }
 
-__device__ __forceinline__ void accumulate(double2* o, double w, double re, double im)
+// the wave sums of one slot, wave-uniform
+__device__ __forceinline__ double2 slot_sum(double re, double im) { return make_double2(wave_sum(re), wave_sum(im)); }
+
+// lane k of v takes the wave-uniform s
+__device__ __forceinline__ void put_lane(double2& v, int k, double2 s)
 {
-    const double r = wave_sum(re), i = wave_sum(im);
-    if ((threadIdx.x & 63) == 0)
-        *o = make_double2(fma(w, r, o->x), fma(w, i, o->y));
+    if ((int)(threadIdx.x & 63) == k)
+        v = s;
 }
 
 // Work split as the stage kernel: workgroup = tile of 64 ADOs (lane = ADO) and one
@@ -118,6 +140,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
 
     double2 dsum = make_double2(0.0, 0.0);
     double2* out = a.part + ((int64_t)tile * kPairs + pr) * (N * N * SLOTS);
+    static_assert(SLOTS <= 64, "one lane per slot of an element");
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         const int j = jb + c;
@@ -125,7 +148,10 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
             break;
         const int e = i * N + j;
         const double2* plane = Y + (int64_t)e * mp;
-        double2* o = out + e * SLOTS;
+        // the element's SLOTS sums (consecutive in `out`), one per lane, added to `out`
+        // with one read-modify-write
+        double2 lanes = make_double2(0.0, 0.0);
+        auto put = [&](int k, double2 v) { put_lane(lanes, k, v); };
         // neighbour loads of the first term block go out before the LDS work
         int2 nbv[TB];
         double2 xp[TB], xm[TB];
@@ -148,8 +174,8 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
         const double2 d = cdot(mm[c], ym[c]);
         dsum.x += d.x;
         dsum.y += d.y;
-        accumulate(o, w, tt.x, tt.y);
-        accumulate(o + 1, w, d.x, d.y);
+        put(0, slot_sum(tt.x, tt.y));
+        put(1, slot_sum(d.x, d.y));
 #pragma unroll
         for (int k0 = 0; k0 < NT; k0 += TB) {
             if (k0 > 0) {
@@ -168,9 +194,13 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
                 const double sp = sqrt((double)(v + 1)), sn = sqrt((double)v);
                 const double2 pp = cdot(mm[c], xp[k]);  // times i sqrt(n+1)
                 const double2 m = cdot(mm[c], xm[k]);
-                accumulate(o + 2 + k0 + k, w, -sp * pp.y, sp * pp.x);
-                accumulate(o + 2 + NT + k0 + k, w, sn * m.x, sn * m.y);
+                put(2 + k0 + k, slot_sum(-sp * pp.y, sp * pp.x));
+                put(2 + NT + k0 + k, slot_sum(sn * m.x, sn * m.y));
             }
+        }
+        if (lane < SLOTS) {
+            double2* o = out + e * SLOTS + lane;
+            *o = make_double2(fma(w, lanes.x, o->x), fma(w, lanes.y, o->y));
         }
     }
     red[wv * kTile + lane] = dsum;
