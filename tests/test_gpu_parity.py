@@ -253,3 +253,25 @@ def test_errors_are_reported(oracle_mod):
             h.set_state(np.zeros(3))
         with pytest.raises(heom_amd.HeomError):
             h.set_mask(np.full(inst.matrices, 7, np.uint8))
+
+
+@pytest.mark.parametrize("D", [1, 2, 62])
+def test_rhs_single_mode_edge_depths(oracle_mod, stage_kernel, D):
+    """One bath mode (W = 1): M = D + 1 ADOs, down to 2; depth 62 reaches the largest
+    tuple entry the kernels take (their sqrt(n_m + 1) table has 64 entries).  State 1
+    couples to no bath."""
+    inst = make_instance(3, 1, 1, D, [[0], [-1], [0]], seed=D)
+    assert inst.matrices == D + 1
+    check_rhs(inst, seed=D, kernel=stage_kernel)
+    y = random_state(inst, D)
+    ref = inst.propagate(y.copy(), 1e-15, 20, 5)
+    with heom_amd.Handle.from_instance(inst) as h:
+        h.set_state(y)
+        tops = h.propagate(1e-15, 20, 5)
+    assert max_abs_rel(tops, ref) < 1e-10
+
+
+def test_tuple_entry_above_62_is_refused(oracle_mod):
+    inst = make_instance(2, 1, 1, 63, [[0], [0]], seed=1)
+    with pytest.raises(heom_amd.HeomError, match="62"):
+        heom_amd.Handle.from_instance(inst)
