@@ -1495,6 +1495,30 @@ int heom_propagate_timedep(heom_handle* h, double dt, int64_t nsteps, int64_t ob
     });
 }
 
+int heom_observe(heom_handle* h, const double* ops, double* pops, double* traces)
+{
+    return guard([&] {
+        need(h);
+        if (!pops)
+            raise(HEOM_E_INVALID, "pops is NULL");
+        if (ops && !traces)
+            raise(HEOM_E_INVALID, "traces is NULL while ops is given");
+        DevBuf<double> p;
+        DevBuf<double2> t, o;
+        p.alloc((size_t)h->count * h->N);
+        if (ops) {
+            t.alloc((size_t)h->count);
+            o.alloc((size_t)h->count * h->nel);
+            HIPC(hipMemcpyAsync(o.p, ops, o.bytes(), hipMemcpyHostToDevice, h->stream));
+        }
+        observe_launch(h, ops ? o.p : nullptr, p.p, ops ? t.p : nullptr);
+        HIPC(hipMemcpyAsync(pops, p.p, p.bytes(), hipMemcpyDeviceToHost, h->stream));
+        if (ops)
+            HIPC(hipMemcpyAsync(traces, t.p, t.bytes(), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+    });
+}
+
 int heom_propagate_observe(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps, const double* ops,
                            double* pops, double* traces)
 {

@@ -39,7 +39,7 @@ EXPORTED = [
     "heom_propagate", "heom_mmult_left", "heom_sync", "heom_timing", "heom_device_bytes",
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
-    "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados",
+    "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados", "heom_observe",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -136,6 +136,7 @@ def load():
             "heom_create_tss": [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
             "heom_tss_step": [vp, i64],
             "heom_get_ados": [vp, ctypes.c_int32, vp, i64, vp],
+            "heom_observe": [vp, vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -351,6 +352,17 @@ class Handle:
         out = np.empty(self.state_size)
         _check(load().heom_get_state(self._h, out.ctypes.data))
         return out
+
+    def observe(self, ops=None):
+        """(populations [count, N], traces [count] complex or None) of the current state."""
+        pops = np.zeros(self.count * self.states)
+        if ops is None:
+            _check(load().heom_observe(self._h, None, pops.ctypes.data, None))
+            return pops.reshape(self.count, self.states), None
+        o = self._per_instance(ops)
+        tr = np.zeros(self.count * 2)
+        _check(load().heom_observe(self._h, o.ctypes.data, pops.ctypes.data, tr.ctypes.data))
+        return pops.reshape(self.count, self.states), tr[0::2] + 1j * tr[1::2]
 
     def get_ados(self, tuples, instance: int = 0) -> np.ndarray:
         """The ADOs with the given tuples ([n, W] ints) of one instance: complex [n, N, N]."""

@@ -168,6 +168,12 @@ int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observ
  * FP32 storage, on the FP64 top), one wavefront per instance. */
 int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                            const double* ops, double* pops, double* traces);
+/* The observations of the current state, as heom_propagate_observe records them: the
+ * populations Re sigma_0[i][i] (matrix_diagonal_observer.hpp:28-34) into pops
+ * [count][N] and, when ops [count][N*N*2] is given, Tr(ops_k sigma_0)
+ * (matrix_trace_observer.hpp:139-158) into traces [count][2]; FP64 reductions on the
+ * device. */
+int heom_observe(heom_handle* handle, const double* ops, double* pops, double* traces);
 /* A time-dependent Hamiltonian: the reference's e_field pre-evaluate action
  * (dm-heom/include/heom/e_field.hpp:64-87, hooked in at include/heom/ode.hpp:57-59) sets
  * H = H0 - E(t) D- - conj(E(t)) D+ before every RHS evaluation, at that evaluation's
