@@ -95,8 +95,8 @@ struct AdjointState {
 
 // The thermal state search (tss.hip): the constants of the per-ADO transform.
 struct TssState {
-    DevBuf<double2> c, ct;  // C and C^T [nel]
-    DevBuf<double2> w;      // (i/hbar)(E_i - E_j) [nel]
+    DevBuf<double2> c, ct;  // C and C^T [count][nel]
+    DevBuf<double2> w;      // (i/hbar)(E_i - E_j) [count][nel]
     DevBuf<double2> epsm;   // eps - S_u per slot [Mp]
     bool real_c = false;    // C has no imaginary parts (real symmetric H)
 };

@@ -19,6 +19,7 @@
 // and in four passes over out-of-place planes for 8 < N <= 32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -44,11 +45,12 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 d)  // a / d
 struct TssArgs {
     const double2* __restrict__ k;  // f_TSS(y), planes
     double2* __restrict__ y;        // result, planes
-    const double2* __restrict__ c;
+    const double2* __restrict__ c;  // [count][nel] (c, ct, w): the instance of each tile
     const double2* __restrict__ ct;
     const double2* __restrict__ w;
     const double2* __restrict__ epsm;
-    const int32_t* __restrict__ q2u;  // < 0: padding or the zero slot, kept at zero
+    const int32_t* __restrict__ q2u;        // < 0: padding or the zero slot, kept at zero
+    const int32_t* __restrict__ tile_inst;  // instance of each tile (null: one instance)
     int64_t mp;
 };
 
@@ -73,6 +75,10 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
     const int lane = threadIdx.x & 63;
     const int i = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t q = (int64_t)blockIdx.x * kTile + lane;
+    const int64_t off = a.tile_inst ? (int64_t)a.tile_inst[blockIdx.x] * N * N : 0;
+    const double2* __restrict__ C = a.c + off;
+    const double2* __restrict__ CT = a.ct + off;
+    const double2* __restrict__ Wd = a.w + off;
 #pragma unroll
     for (int j = 0; j < N; ++j)
         t[(i * N + j) * kTile + lane] = a.k[(int64_t)(i * N + j) * a.mp + q];
@@ -83,7 +89,7 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         r[j] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int m = 0; m < N; ++m) {  // r = row i of C k
-        const double2 cim = a.c[i * N + m];
+        const double2 cim = C[i * N + m];
 #pragma unroll
         for (int j = 0; j < N; ++j)
             r[j] = cmac<REALC>(cim, t[(m * N + j) * kTile + lane], r[j]);
@@ -95,8 +101,8 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         double2 v = make_double2(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < N; ++m)
-            v = cmac<REALC>(a.ct[m * N + j], r[m], v);
-        const double2 wij = a.w[i * N + j];
+            v = cmac<REALC>(CT[m * N + j], r[m], v);
+        const double2 wij = Wd[i * N + j];
         s[j] = cdiv(v, make_double2(wij.x + em.x, wij.y + em.y));
     }
     __syncthreads();  // every wavefront is done with k
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         r[j] = make_double2(0.0, 0.0);
 #pragma unroll
     for (int m = 0; m < N; ++m) {  // r = row i of C^T B
-        const double2 ctim = a.ct[i * N + m];
+        const double2 ctim = CT[i * N + m];
 #pragma unroll
         for (int j = 0; j < N; ++j)
             r[j] = cmac<REALC>(ctim, t[(m * N + j) * kTile + lane], r[j]);
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
         double2 v = make_double2(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < N; ++m)
-            v = cmac<REALC>(a.c[m * N + j], r[m], v);
+            v = cmac<REALC>(C[m * N + j], r[m], v);
         a.y[(int64_t)(i * N + j) * a.mp + q] = pad ? make_double2(0.0, 0.0) : v;
     }
 }
@@ -129,7 +135,8 @@ __global__ __launch_bounds__(64 * N) void tss_transform(TssArgs a)
 //   right: out[e][q] = sum_m in[i*N+m][q] mx[m][j]   (divided by D when w is given)
 __global__ void tss_pass(const double2* __restrict__ in, double2* __restrict__ out, const double2* __restrict__ mx,
                          const double2* __restrict__ w, const double2* __restrict__ epsm,
-                         const int32_t* __restrict__ q2u, int64_t mp, int N, int left)
+                         const int32_t* __restrict__ q2u, const int32_t* __restrict__ tile_inst, int64_t mp, int N,
+                         int left)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nel = N * N;
@@ -138,6 +145,12 @@ __global__ void tss_pass(const double2* __restrict__ in, double2* __restrict__ o
     const int e = (int)(t / mp);
     const int64_t q = t % mp;
     const int i = e / N, j = e % N;
+    if (tile_inst) {
+        const int64_t off = (int64_t)tile_inst[q / kTile] * nel;
+        mx += off;
+        if (w)
+            w += off;
+    }
     double2 v = make_double2(0.0, 0.0);
     if (left) {
         for (int m = 0; m < N; ++m)
@@ -182,8 +195,8 @@ void reject_tss(heom_handle* h, const char* what)
 
 extern "C" {
 
-int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenvalues, const double* epsilon,
-                    heom_handle** out)
+int heom_create_tss_batch(const heom_desc* descs, int32_t count, const double* c, const double* eigenvalues,
+                          const double* epsilon, heom_handle** out)
 {
     if (!out) {
         g_error = "out is NULL";
@@ -192,66 +205,86 @@ int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenv
     *out = nullptr;
     heom_handle* h = nullptr;
     const int rc = guard([&] {
-        if (!desc || !c || !eigenvalues || !epsilon)
-            raise(HEOM_E_INVALID, "desc, c, eigenvalues and epsilon must not be NULL");
-        if (desc->precision != HEOM_PRECISION_FP64)
-            raise(HEOM_E_UNSUPPORTED, "the thermal state search runs in FP64 only");
-        const int N = desc->states;
-        const int64_t M = desc->matrices;
-        if (N < 1 || M < 1)
-            raise(HEOM_E_INVALID, "states and matrices must be >= 1");
+        if (!descs || !c || !eigenvalues || !epsilon)
+            raise(HEOM_E_INVALID, "descs, c, eigenvalues and epsilon must not be NULL");
+        if (count < 1)
+            raise(HEOM_E_INVALID, "count must be >= 1");
+        const int N = descs[0].states;
+        if (N < 1)
+            raise(HEOM_E_INVALID, "states must be >= 1");
         if (N > 32)
             raise(HEOM_E_UNSUPPORTED, "more than 32 states");
+        for (int k = 0; k < count; ++k) {
+            if (descs[k].precision != HEOM_PRECISION_FP64)
+                raise(HEOM_E_UNSUPPORTED, "the thermal state search runs in FP64 only");
+            if (descs[k].states != N || descs[k].matrices < 1)
+                raise(HEOM_E_INVALID, "every instance needs the same states and matrices >= 1");
+        }
         const std::complex<double> eps(epsilon[0], epsilon[1]);
         if (!std::isfinite(eps.real()) || !std::isfinite(eps.imag()) || eps == 0.0)
             raise(HEOM_E_INVALID, "epsilon must be finite and non-zero");
-        for (int e = 0; e < 2 * N * N; ++e)
+        const int nel = N * N;
+        for (int64_t e = 0; e < (int64_t)count * 2 * nel; ++e)
             if (!std::isfinite(c[e]))
                 raise(HEOM_E_INVALID, "c must be finite");
-        for (int e = 0; e < 2 * N; ++e)
+        for (int64_t e = 0; e < (int64_t)count * 2 * N; ++e)
             if (!std::isfinite(eigenvalues[e]))
                 raise(HEOM_E_INVALID, "eigenvalues must be finite");
         // the f_TSS operator: H = 0 and S_u = eps (thermal_state_search_solver.hpp:199-209,
         // the pseudo-Hamiltonian eps in every element through heom_ode.cl:116-121)
-        heom_desc d = *desc;
-        std::vector<double> zero_h((size_t)(2 * N * N), 0.0), pss((size_t)(2 * M));
-        for (int64_t u = 0; u < M; ++u) {
-            pss[2 * u] = eps.real();
-            pss[2 * u + 1] = eps.imag();
-        }
-        d.hamiltonian = zero_h.data();
-        d.pf_same_sum = pss.data();
-        h = new heom_handle();
-        create_batch(h, &d, 1);
-        auto T = std::make_unique<TssState>();
-        const int nel = N * N;
-        std::vector<double2> cv(nel), ctv(nel), wv(nel);
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) {
-                cv[i * N + j] = make_double2(c[2 * (i * N + j)], c[2 * (i * N + j) + 1]);
-                ctv[i * N + j] = make_double2(c[2 * (j * N + i)], c[2 * (j * N + i) + 1]);
-                // (i / hbar)(E_i - E_j) as thermal_state_search.cl:45 forms it
-                const double der = eigenvalues[2 * i] - eigenvalues[2 * j];
-                const double dei = eigenvalues[2 * i + 1] - eigenvalues[2 * j + 1];
-                wv[i * N + j] = make_double2(-dei / kHbar, der / kHbar);
+        std::vector<heom_desc> d(descs, descs + count);
+        const std::vector<double> zero_h((size_t)(2 * nel), 0.0);
+        std::vector<std::vector<double>> pss((size_t)count);
+        for (int k = 0; k < count; ++k) {
+            pss[(size_t)k].resize((size_t)(2 * d[(size_t)k].matrices));
+            for (int64_t u = 0; u < d[(size_t)k].matrices; ++u) {
+                pss[(size_t)k][(size_t)(2 * u)] = eps.real();
+                pss[(size_t)k][(size_t)(2 * u + 1)] = eps.imag();
             }
-        // eps - S_u per slot, in the handle's ADO order
+            d[(size_t)k].hamiltonian = zero_h.data();
+            d[(size_t)k].pf_same_sum = pss[(size_t)k].data();
+        }
+        h = new heom_handle();
+        create_batch(h, d.data(), count);
+        auto T = std::make_unique<TssState>();
+        std::vector<double2> cv((size_t)count * nel), ctv((size_t)count * nel), wv((size_t)count * nel);
+        for (int k = 0; k < count; ++k) {
+            const double* ck = c + (size_t)k * 2 * nel;
+            const double* ek = eigenvalues + (size_t)k * 2 * N;
+            double2* cvk = &cv[(size_t)k * nel];
+            double2* ctk = &ctv[(size_t)k * nel];
+            double2* wk = &wv[(size_t)k * nel];
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    cvk[i * N + j] = make_double2(ck[2 * (i * N + j)], ck[2 * (i * N + j) + 1]);
+                    ctk[i * N + j] = make_double2(ck[2 * (j * N + i)], ck[2 * (j * N + i) + 1]);
+                    // (i / hbar)(E_i - E_j) as thermal_state_search.cl:45 forms it
+                    const double der = ek[2 * i] - ek[2 * j];
+                    const double dei = ek[2 * i + 1] - ek[2 * j + 1];
+                    wk[i * N + j] = make_double2(-dei / kHbar, der / kHbar);
+                }
+        }
+        // eps - S_u per slot, in the handle's ADO order (q2u holds batch-global uids)
         std::vector<int32_t> q2u((size_t)h->Mp);
         HIPC(hipMemcpyAsync(q2u.data(), h->q2u.p, q2u.size() * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         std::vector<double2> em((size_t)h->Mp, make_double2(eps.real(), eps.imag()));
         for (int64_t q = 0; q < h->Mp; ++q) {
             const int32_t u = q2u[(size_t)q];
-            if (u >= 0)
-                em[(size_t)q] = make_double2(eps.real() - desc->pf_same_sum[2 * u], eps.imag() - desc->pf_same_sum[2 * u + 1]);
+            if (u < 0)
+                continue;
+            const int k = (int)(std::upper_bound(h->inst_u.begin(), h->inst_u.end(), (int64_t)u) - h->inst_u.begin()) - 1;
+            const int64_t lu = u - h->inst_u[(size_t)k];
+            em[(size_t)q] = make_double2(eps.real() - descs[k].pf_same_sum[2 * lu],
+                                         eps.imag() - descs[k].pf_same_sum[2 * lu + 1]);
         }
         auto up = [&](DevBuf<double2>& b, const std::vector<double2>& v) {
             b.alloc(v.size());
             HIPC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
         };
         T->real_c = true;
-        for (int e = 0; e < nel; ++e)
-            T->real_c = T->real_c && cv[e].y == 0.0;
+        for (const double2& x : cv)
+            T->real_c = T->real_c && x.y == 0.0;
         up(T->c, cv);
         up(T->ct, ctv);
         up(T->w, wv);
@@ -267,6 +300,12 @@ int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenv
     return HEOM_OK;
 }
 
+int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenvalues, const double* epsilon,
+                    heom_handle** out)
+{
+    return heom_create_tss_batch(desc, 1, c, eigenvalues, epsilon, out);
+}
+
 int heom_tss_step(heom_handle* h, int64_t n)
 {
     return guard([&] {
@@ -279,7 +318,8 @@ int heom_tss_step(heom_handle* h, int64_t n)
         double2* y = h->y.as<double2>();
         double2* ya = h->ya.as<double2>();
         double2* yb = h->yb.as<double2>();
-        TssArgs a{ya, y, T.c.p, T.ct.p, T.w.p, T.epsm.p, h->q2u.p, h->Mp};
+        const int32_t* ti = h->count > 1 ? h->tile_inst.p : nullptr;
+        TssArgs a{ya, y, T.c.p, T.ct.p, T.w.p, T.epsm.p, h->q2u.p, ti, h->Mp};
         const int64_t total = (int64_t)h->nel * h->Mp;
         const dim3 pgrid(blocks_for(total, 256)), pblock(256);
         for (int64_t s = 0; s < n; ++s) {
@@ -295,10 +335,10 @@ int heom_tss_step(heom_handle* h, int64_t n)
             case 8: launch_transform<8>(h, a, T.real_c); break;
             default:
                 // C k -> yb, (. C^T) / D -> ya, C^T . -> yb, . C -> y, in the reference's order
-                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, ya, yb, T.c.p, nullptr, nullptr, h->q2u.p, h->Mp, N, 1);
-                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, yb, ya, T.ct.p, T.w.p, T.epsm.p, h->q2u.p, h->Mp, N, 0);
-                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, ya, yb, T.ct.p, nullptr, nullptr, h->q2u.p, h->Mp, N, 1);
-                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, yb, y, T.c.p, nullptr, nullptr, h->q2u.p, h->Mp, N, 0);
+                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, ya, yb, T.c.p, nullptr, nullptr, h->q2u.p, ti, h->Mp, N, 1);
+                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, yb, ya, T.ct.p, T.w.p, T.epsm.p, h->q2u.p, ti, h->Mp, N, 0);
+                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, ya, yb, T.ct.p, nullptr, nullptr, h->q2u.p, ti, h->Mp, N, 1);
+                hipLaunchKernelGGL(tss_pass, pgrid, pblock, 0, h->stream, yb, y, T.c.p, nullptr, nullptr, h->q2u.p, ti, h->Mp, N, 0);
                 break;
             }
             HIPC(hipGetLastError());

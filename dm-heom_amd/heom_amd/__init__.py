@@ -40,6 +40,7 @@ EXPORTED = [
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
     "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados", "heom_observe",
+    "heom_create_tss_batch",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -135,6 +136,7 @@ def load():
             "heom_propagate_timedep": [vp, dbl, i64, i64, vp, vp],
             "heom_create_tss": [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
             "heom_tss_step": [vp, i64],
+            "heom_create_tss_batch": [vp, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
             "heom_get_ados": [vp, ctypes.c_int32, vp, i64, vp],
             "heom_observe": [vp, vp, vp, vp],
         }
@@ -276,6 +278,43 @@ class Handle:
         self.precision = desc.precision
         self._keep = keepalive
         return self
+
+    @classmethod
+    def thermal_state_search_batch(cls, descs, cs, eigenvalues, epsilon, keepalive=None) -> "Handle":
+        """heom_create_tss_batch: one search handle over several instances (same states and
+        baths), with one C and eigenvalue set per instance and a common epsilon."""
+        self = cls.__new__(cls)
+        n = len(descs)
+        N = int(descs[0].states)
+        cc = _f64(np.stack([np.stack([np.real(c), np.imag(c)], -1).reshape(-1) for c in cs]))
+        ev = _f64(np.stack([np.stack([np.real(e), np.imag(e)], -1).reshape(-1) for e in eigenvalues]))
+        if cc.size != n * 2 * N * N or ev.size != n * 2 * N:
+            raise ValueError("one N x N C and N eigenvalues per instance")
+        eps = np.array([complex(epsilon).real, complex(epsilon).imag])
+        arr = (Desc * n)(*descs)
+        h = ctypes.c_void_p()
+        _check(load().heom_create_tss_batch(arr, n, cc.ctypes.data, ev.ctypes.data, eps.ctypes.data,
+                                            ctypes.byref(h)))
+        self._h = h
+        self.count = n
+        self.states = N
+        self.instance_matrices = [int(d.matrices) for d in descs]
+        self.matrices = sum(self.instance_matrices)
+        self.width = descs[0].width
+        self.baths = descs[0].baths
+        self.matsubaras = descs[0].matsubaras
+        self.precision = descs[0].precision
+        self._keep = keepalive
+        return self
+
+    @classmethod
+    def thermal_state_search_from_instances(cls, insts, setups) -> "Handle":
+        """A batched search handle over instances' arrays with one setup (c, eigenvalues,
+        epsilon) each."""
+        made = [_make_desc(precision=HEOM_PRECISION_FP64, **_instance_arrays(i)) for i in insts]
+        return cls.thermal_state_search_batch([d for d, _ in made], [s["c"] for s in setups],
+                                              [s["eigenvalues"] for s in setups], setups[0]["epsilon"],
+                                              [k for _, k in made])
 
     @classmethod
     def thermal_state_search_from_instance(cls, inst, setup: dict) -> "Handle":

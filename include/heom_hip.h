@@ -196,6 +196,11 @@ int heom_propagate_timedep(heom_handle* handle, double h, int64_t nsteps, int64_
  * Hamiltonian updates, compaction and the adjoint are refused on a search handle. */
 int heom_create_tss(const heom_desc* desc, const double* c, const double* eigenvalues, const double* epsilon,
                     heom_handle** out);
+/* The search for `count` instances at once (as heom_create_batch: same states and baths,
+ * any depth): c [count][N*N*2], eigenvalues [count][N*2], one epsilon.  Every
+ * heom_tss_step iterates all of them; heom_create_tss is the batch of one. */
+int heom_create_tss_batch(const heom_desc* descs, int32_t count, const double* c, const double* eigenvalues,
+                          const double* epsilon, heom_handle** out);
 int heom_tss_step(heom_handle* handle, int64_t n);
 /* solver_base::hierarchy_mmult_left (solver_base.hpp:250-257; kernel
  * cl/heom/hierarchy_arithmetic_mmult_left.cl:15-50): sigma_n <- mx * sigma_n */

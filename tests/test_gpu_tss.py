@@ -137,3 +137,35 @@ def test_complex_transform_matches_oracle(oracle_mod, name, depth):
     err = max_abs_rel(got, ref)
     print(f"{name or 'fmo_thermal_state'}: complex C, 3 steps {err:.1e}", flush=True)
     assert err < 1e-10
+
+
+def test_batched_search_equals_single_handles_and_oracle(oracle_mod):
+    """Three static-disorder-like instances (different H, depths 2-4) in one search handle:
+    every instance's state equals its own single-instance search bit for bit and the
+    oracle's to 1e-10, for the fused (N = 8) and the four-pass (N = 24) transform."""
+    for name, depths in (("excitonic8_d6", (2, 3, 4)), ("excitonic24_d4", (1, 2, 1))):
+        cases = []
+        for k, D in enumerate(depths):
+            cfg, inst = instance(name, depth=D)
+            H = inst.hamiltonian_matrix()
+            H = H + np.diag(np.random.default_rng(k).normal(0, 50, inst.states)) * O.INVCM_TO_JOULE
+            s = O.thermal_state_setup(H, cfg.baths_temperature)
+            inst.set_hamiltonian(s["pseudo_hamiltonian"])
+            cases.append((inst, s, random_state(inst, 10 + k)))
+        with heom_amd.Handle.thermal_state_search_from_instances([c[0] for c in cases],
+                                                                 [c[1] for c in cases]) as hb:
+            hb.set_state(np.concatenate([c[2] for c in cases]))
+            hb.tss_step(4)
+            sb = hb.get_state()
+        off = 0
+        for inst, s, y in cases:
+            with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+                h.set_state(y)
+                h.tss_step(4)
+                one = h.get_state()
+            ref = y.copy()
+            inst.tss_step(ref, s, 4)
+            n = inst.state_size
+            assert np.array_equal(sb[off:off + n], one), name
+            assert max_abs_rel(one, ref) < 1e-10, name
+            off += n
