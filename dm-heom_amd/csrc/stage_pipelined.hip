@@ -145,6 +145,12 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
         }
     };
 
+    // the epilogue operands (y_n or P) of all columns go out first: they come from HBM
+    // and are needed only at the end of each column
+    C epv[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+        epv[c] = stage_operand<STAGE, R>(a, (int64_t)(i * N + col(c)) * mp + q);
     load(0);
     __builtin_amdgcn_sched_barrier(0);
     issue();
@@ -154,7 +160,7 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
     for (int c = 0; c < CW; ++c) {
         const int j = col(c);
         const int e = i * N + j;
-        const C ep = stage_operand<STAGE, R>(a, (int64_t)e * mp + q);
+        const C ep = epv[c];
         if (c > 0)
             load(c);
         const TermT<R>* te = opaque(terms) + e * NT;
