@@ -47,6 +47,9 @@ def parse_args():
                         "adjoint, filtering (compaction), field (time-dependent H), tss (thermal state search)")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    p.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                   help="state storage of the timed run: the plain FP64 path (the headline figure) or FP32 "
+                        "storage with FP64 tier-0/1 ADOs")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel of "
                         "known size, so that profiler byte counters can be calibrated in the same run)")
@@ -428,7 +431,8 @@ def main():
     cfg_path = os.path.join(heom_amd.CONFIG_DIR, f"{args.config}.cfg")
     host = heom_amd.HostInstance(cfg_path, task=heom_amd.TASK_POPULATION_DYNAMICS)
     info = host.info
-    h = host.create_handle()
+    fp32 = args.precision == "fp32"
+    h = host.create_handle(precision=heom_amd.HEOM_PRECISION_FP32 if fp32 else heom_amd.HEOM_PRECISION_FP64)
     M, N = host.matrices, host.states
     dt = info.step_size
     tuning = h.autotune(steps=3)  # the stage-kernel configuration, timed on this hierarchy
@@ -456,13 +460,13 @@ def main():
     pcie = pcie_inclusive(h, dt, args.steps, M, wall)
 
     value = M * args.steps / wall
-    bytes_per_ado_step = 16 * N * N * 16  # 16 state passes of N^2 complex128 (algorithmic)
+    bytes_per_ado_step = 16 * N * N * (8 if fp32 else 16)  # 16 state passes of N^2 complex (algorithmic)
     launches = 4 * args.steps
     per_launch_s = ev / launches
     alg_bytes_per_launch = bytes_per_ado_step * M / 4
     achieved = alg_bytes_per_launch / per_launch_s / 1e9
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
+    traffic = None  # measured for the FP64 kernels (profiles/), not for FP32 storage
+    if not fp32 and os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
@@ -477,11 +481,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 (FP64 tier-0/1 ADOs and reductions)" if fp32 else "f64",
         "data": "synthetic (seeded config file, no external data)",
         "config": {
             "workload": f"{args.config}: {info.sites} sites, {info.width} bath modes, depth {info.depth}, "
-                        f"{M} complex128 {N}x{N} ADOs, classical RK4 dt={dt:g} s",
+                        f"{M} {'complex64' if fp32 else 'complex128'} {N}x{N} ADOs, classical RK4 dt={dt:g} s",
             "ados": M,
             "states": N,
             "modes": info.width,
