@@ -131,17 +131,33 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
     // and are not stored: every wavefront runs the same straight-line code, which keeps
     // the per-column arrays in registers.
     auto col = [&](int c) { return min(jb + c, N - 1); };
+    // Neighbour gathers as raw buffer loads over the stage input: an edge to the zero
+    // slot (missing or masked neighbour, two thirds of the lanes at cfg1) gets an offset
+    // past the buffer's size, which returns zero without a memory request.
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)a.yin, (short)0, (int)(N * N * mp * (int64_t)sizeof(C)), 0x00020000);
+    const uint32_t zero = (uint32_t)a.zero_slot;
+    auto gather = [&](uint32_t off_e, uint32_t idx) {
+        const uint32_t vo = idx == zero ? 0x80000000u : off_e + idx * (uint32_t)sizeof(C);
+        if constexpr (sizeof(C) == 16) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, vo, 0, 0);
+            return mk2<R>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, vo, 0, 0);
+            return mk2<R>(__int_as_float((int)v[0]), __int_as_float((int)v[1]));
+        }
+    };
     auto load = [&](int c) {
         const int e = i * N + col(c);
-        const C* plane = a.yin + (int64_t)e * mp;
+        const uint32_t off_e = (uint32_t)(e * mp * (int64_t)sizeof(C));
         const TermT<R>* te = opaque(terms) + e * NT;
 #pragma unroll
         for (int k = 0; k < NT; ++k)
             L[c].nbv[k] = tab[te[k].mode * kTile + lane];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            L[c].xp[k] = plane[(uint32_t)L[c].nbv[k].x >> 6];
-            L[c].xm[k] = plane[(uint32_t)L[c].nbv[k].y];
+            L[c].xp[k] = gather(off_e, (uint32_t)L[c].nbv[k].x >> 6);
+            L[c].xm[k] = gather(off_e, (uint32_t)L[c].nbv[k].y);
         }
     };
 
@@ -336,8 +352,11 @@ bool pipe_supported(int N, int W, int64_t mp, int complex_bytes)
     // N = 8, whose column blocks are full: at N = 6, 7 that kernel spills registers), the
     // table slice fits the LDS buffer, and table byte offsets fit in 32 bits
     const bool shape = complex_bytes == (int)sizeof(double2) ? N >= 6 && N <= kMaxStates : N == kMaxStates;
+    // (and the state below 2 GiB: the gathers are buffer loads whose size and out-of-range
+    // marker are 32-bit)
     return shape && ((W + 2) & ~1) <= pipe_table_capacity(N, complex_bytes) &&
-           (int64_t)(W + 2) * mp * (int64_t)sizeof(int2) < (int64_t(1) << 32);
+           (int64_t)(W + 2) * mp * (int64_t)sizeof(int2) < (int64_t(1) << 32) &&
+           (int64_t)N * N * mp * complex_bytes < (int64_t(1) << 31);
 }
 
 void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
