@@ -115,3 +115,34 @@ def test_thermal_state_search_app_fmo(oracle_mod, tmp_path):
     assert err < 1e-10
     _, tr = read_table(tmp_path / "tss_trace.dat")
     assert np.max(np.abs(tr[:, 1] - 1.0)) < 1e-9
+
+
+def test_thermal_state_search_app_filtered_with_norms(oracle_mod, tmp_path):
+    """The app with the single-excitation filter (first layer 1) and a hierarchy_norm
+    observation: the last row's populations and ADO norms against the oracle's search
+    with the host library's mask, replayed for as many iterations as the app wrote."""
+    heom_amd = pytest.importorskip("heom_amd")
+    cfg_path = write_cfg(os.path.join(FMO_DIR, "fmo_thermal_state_77K.cfg"), tmp_path / "tssf.cfg",
+                         observations="{(matrix_diagonal, f_diag.dat), (hierarchy_norm, f_norm.dat)}",
+                         strategy="single_excitation", first_layer="1", max_steps="200", steps="100")
+    r = run_app("heom_thermal_state_search", str(cfg_path), tmp_path)
+    assert "thermal state search:" in r.stdout
+    _, got = read_table(tmp_path / "f_diag.dat")
+    cfg = O.parse_config(str(cfg_path))
+    inst = O.instance_from_config(cfg)
+    mask = heom_amd.HostInstance(str(cfg_path), task=heom_amd.TASK_THERMAL_STATE_SEARCH).mask()
+    s = O.thermal_state_setup(inst.hamiltonian_matrix(), cfg.baths_temperature)
+    inst.set_hamiltonian(s["pseudo_hamiltonian"])
+    inst.mask[:] = mask
+    y = inst.state_with_top(s["top"])
+    rows = got.shape[0] - 1
+    inst.tss_step(y, s, rows * cfg.observe_steps)
+    pops = np.real(np.diag(inst.top(y)))
+    err = np.max(np.abs(got[-1, 1:] - pops))
+    print(f"filtered FMO thermal state: {rows} iterations, population deviation {err:.1e}", flush=True)
+    assert err < 1e-10
+    _, norms = read_table(tmp_path / "f_norm.dat")
+    assert norms.shape == (got.shape[0], inst.matrices + 1)
+    ref = inst.norms(y)
+    nerr = np.max(np.abs(norms[-1, 1:] - ref)) / np.max(ref)
+    assert nerr < 1e-10
