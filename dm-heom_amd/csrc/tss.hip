@@ -219,6 +219,9 @@ int heom_create_tss_batch(const heom_desc* descs, int32_t count, const double* c
                 raise(HEOM_E_UNSUPPORTED, "the thermal state search runs in FP64 only");
             if (descs[k].states != N || descs[k].matrices < 1)
                 raise(HEOM_E_INVALID, "every instance needs the same states and matrices >= 1");
+            // read below for eps - S_u; the descriptor handed on carries replacements
+            if (!descs[k].pf_same_sum)
+                raise(HEOM_E_INVALID, "desc has a NULL array (pf_same_sum)");
         }
         const std::complex<double> eps(epsilon[0], epsilon[1]);
         if (!std::isfinite(eps.real()) || !std::isfinite(eps.imag()) || eps == 0.0)

@@ -52,3 +52,27 @@ def test_apps_print_usage():
         r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0
         assert "Usage" in r.stderr
+
+
+def test_search_handle_checks_without_gpu():
+    """heom_create_tss refuses bad arguments before any device call: a NULL S_u array
+    (read on the host for eps - S_u), FP32, a zero epsilon."""
+    L = heom_amd.load()
+    N = 2
+    c = (ctypes.c_double * (2 * N * N))(1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    ev = (ctypes.c_double * (2 * N))()
+    eps = (ctypes.c_double * 2)(1e14, 0.0)
+    h = ctypes.c_void_p()
+    d = heom_amd.Desc(states=N, matsubaras=1, baths=1, baths_max_per_state=1, width=1,
+                      precision=heom_amd.HEOM_PRECISION_FP64, matrices=3)
+    rc = L.heom_create_tss(ctypes.byref(d), c, ev, eps, ctypes.byref(h))
+    assert rc == heom_amd.HEOM_E_INVALID and b"pf_same_sum" in L.heom_last_error()
+    assert not h.value
+    d.precision = heom_amd.HEOM_PRECISION_FP32
+    assert L.heom_create_tss(ctypes.byref(d), c, ev, eps, ctypes.byref(h)) == heom_amd.HEOM_E_UNSUPPORTED
+    d.precision = heom_amd.HEOM_PRECISION_FP64
+    pss = (ctypes.c_double * 6)()
+    d.pf_same_sum = ctypes.cast(pss, ctypes.c_void_p)
+    zero = (ctypes.c_double * 2)()
+    assert L.heom_create_tss(ctypes.byref(d), c, ev, zero, ctypes.byref(h)) == heom_amd.HEOM_E_INVALID
+    assert b"epsilon" in L.heom_last_error()
