@@ -169,3 +169,38 @@ def test_batched_search_equals_single_handles_and_oracle(oracle_mod):
             assert np.array_equal(sb[off:off + n], one), name
             assert max_abs_rel(one, ref) < 1e-10, name
             off += n
+
+
+def test_masked_batched_search_equals_masked_single_handles(oracle_mod):
+    """A mask over a batched search handle (the instances' masks concatenated) acts on
+    each instance as on its own handle: bitwise the single masked searches, and the
+    oracle's masked search to 1e-10 on the ADOs the mask keeps (as for one instance)."""
+    cases = []
+    for k, D in enumerate((3, 2)):
+        cfg, inst = instance("excitonic8_d6", depth=D)
+        s = O.thermal_state_setup(inst.hamiltonian_matrix(), cfg.baths_temperature)
+        inst.set_hamiltonian(s["pseudo_hamiltonian"])
+        rng = np.random.default_rng(20 + k)
+        mask = np.where(rng.random(inst.matrices) < 0.3, O.MASK_FALSE, O.MASK_TRUE).astype(np.uint8)
+        mask[0] = O.MASK_PROTECTED
+        cases.append((inst, s, mask, random_state(inst, 30 + k)))
+    with heom_amd.Handle.thermal_state_search_from_instances([c[0] for c in cases], [c[1] for c in cases]) as hb:
+        hb.set_mask(np.concatenate([c[2] for c in cases]))
+        hb.set_state(np.concatenate([c[3] for c in cases]))
+        hb.tss_step(3)
+        sb = hb.get_state()
+    off = 0
+    for inst, s, mask, y in cases:
+        with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+            h.set_mask(mask)
+            h.set_state(y)
+            h.tss_step(3)
+            one = h.get_state()
+        ref = y.copy()
+        inst.mask[:] = mask
+        inst.tss_step(ref, s, 3)
+        n = inst.state_size
+        keep = np.repeat(mask != O.MASK_FALSE, inst.states * inst.states * 2)
+        assert np.array_equal(sb[off:off + n], one)
+        assert max_abs_rel(one[keep], ref[keep]) < 1e-10
+        off += n
