@@ -1,8 +1,8 @@
 """The thermal state search on the GPU (heom_create_tss / heom_tss_step) against the
 oracle's oracle_tss_step on the same inputs: f_TSS (heom_rhs on a search handle), single
 and repeated iterations for the fused transform (N <= 8) and the four-pass one (N = 24),
-a filtered hierarchy, the full search of the FMO thermal-state config to convergence,
-and the entry points a search handle refuses."""
+a filtered hierarchy, batches, one iteration at 735,471 ADOs, the full search of the FMO
+thermal-state config to convergence, and the entry points a search handle refuses."""
 import numpy as np
 import pytest
 
@@ -204,3 +204,25 @@ def test_masked_batched_search_equals_masked_single_handles(oracle_mod):
         assert np.array_equal(sb[off:off + n], one)
         assert max_abs_rel(one[keep], ref[keep]) < 1e-10
         off += n
+
+
+def test_full_size_iteration_matches_oracle(oracle_mod):
+    """735,471 ADOs (8 states, 16 modes, depth 8): one iteration from a random hierarchy
+    against the oracle, then 5 from the Boltzmann start, whose top keeps trace one and
+    stays Hermitian."""
+    cfg, inst, s = search_case("excitonic8_d8", None)
+    y = random_state(inst, 40)
+    ref = y.copy()
+    inst.tss_step(ref, s, 1)
+    with heom_amd.Handle.thermal_state_search_from_instance(inst, s) as h:
+        h.set_state(y)
+        h.tss_step(1)
+        got = h.get_state()
+        err = max_abs_rel(got, ref)
+        print(f"excitonic8_d8 ({inst.matrices} ADOs): 1 iteration {err:.1e}", flush=True)
+        assert err < 1e-10
+        h.set_state(inst.state_with_top(s["top"]))
+        h.tss_step(5)
+        top = h.get_top()
+    assert abs(np.trace(top) - 1.0) < 1e-12
+    assert np.max(np.abs(top - top.conj().T)) < 1e-12
