@@ -20,7 +20,14 @@
  *                        the exact call sequence of the un-vendored noma/num stepper is
  *                        parity unpinned (classical tableau assumed)
  *   norms, mmult_left,   pinned: textbook definitions checked in numpy
- *   trace
+ *   mmult_right, trace
+ *   propagate_timedep    pinned: a zero field is plain RK4 bit for bit; with a pulse, step
+ *                        halving converges at fourth order to a fine-step reference (fails
+ *                        if a stage sees the Hamiltonian at the wrong time)
+ *   rhs_tss, tss_step    pinned: f_TSS equals eps y plus the ordinary RHS with H = 0 and
+ *                        S_u = 0; the converged search equals the null vector of the dense
+ *                        Liouvillian (SVD), and a planted sign error in the denominator
+ *                        converges elsewhere; the Boltzmann start against scipy's expm
  *   adjoint              no reference counterpart; pinned: <mu, L y> = <L^dag mu, y> on random
  *                        states against the pinned rhs, every coefficient gradient against
  *                        central differences of oracle_propagate, and the discrete-adjoint
