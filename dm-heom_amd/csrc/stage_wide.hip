@@ -16,6 +16,9 @@ namespace heom_hip {
 namespace {
 
 constexpr int kWideLdsBytes = 147456;  // 144 KB of the 160 KB LDS
+// a second instance with 48 KB, chosen when H and the tile fit: then up to three
+// workgroups share a CU instead of one, as far as their registers allow
+constexpr int kWideLdsSmallBytes = 49152;
 constexpr int kMaxWideStates = 32;
 
 // LDS row stride (complex elements) of one N x N matrix: one pad element breaks the
@@ -50,13 +53,13 @@ int wide_tile(int N, int cbytes)
 //      the RK4 epilogue with the own element re-read from global memory (still in L2),
 //      so that loads of the stage operand and all stores move T consecutive ADOs of
 //      one plane together.  LDS: H and one tile, so that several workgroups share a CU.
-template <typename R, int B, int NT, int STAGE, bool REALH>
+template <typename R, int B, int NT, int STAGE, bool REALH, int LDSB>
 __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
                                                                const c2<R>* __restrict__ gam,
                                                                const TermT<R>* __restrict__ terms, int N, int T)
 {
     using C = c2<R>;
-    __shared__ C lds[kWideLdsBytes / sizeof(C)];
+    __shared__ C lds[LDSB / sizeof(C)];
     const int nel = N * N, NP = wide_row(N), mat = N * NP;
     C* hl = lds;         // [N][NP] H / hbar
     C* sig = lds + mat;  // [T][N][NP] the tile, then commutator + same term
@@ -188,17 +191,27 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
     }
 }
 
-template <typename R, int B, int NT, bool REALH>
-void launch_wb(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
+template <typename R, int B, int NT, bool REALH, int LDSB>
+void launch_wl(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
 {
     const dim3 grid((unsigned)(a.mp / T)), block(64 * T);
     switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 0, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
-    case 1: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 1, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
-    case 2: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 2, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
-    case 3: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 3, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
-    default: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 4, REALH>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 0: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 0, REALH, LDSB>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 1: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 1, REALH, LDSB>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 2: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 2, REALH, LDSB>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    case 3: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 3, REALH, LDSB>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
+    default: hipLaunchKernelGGL((rk4_stage_wide<R, B, NT, 4, REALH, LDSB>), grid, block, 0, st, a, el.hs, el.gam, el.terms, N, T); break;
     }
+}
+
+// the LDS instance: H and T matrices of N x wide_row(N) complex elements
+template <typename R, int B, int NT, bool REALH>
+void launch_wb(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
+{
+    if ((int64_t)(T + 1) * N * wide_row(N) * (int64_t)sizeof(c2<R>) <= kWideLdsSmallBytes)
+        launch_wl<R, B, NT, REALH, kWideLdsSmallBytes>(N, stage, a, el, T, st);
+    else
+        launch_wl<R, B, NT, REALH, kWideLdsBytes>(N, stage, a, el, T, st);
 }
 
 template <typename R, int B, bool REALH>
