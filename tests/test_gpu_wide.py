@@ -1,6 +1,8 @@
 """Large systems (8 < N <= 32 states, stage_wide.hip): the 24-site workload against the
-oracle, in FP64 (1e-10 relative) and FP32 storage, and the FP32-storage population
-error against FP64 over 1000 fs at depth 4 (20,475 complex64 24 x 24 ADOs)."""
+oracle, in FP64 (1e-10 relative) and FP32 storage, N = 9, 12, 16 and 32 in FP64 (both
+LDS instances of the kernel), every workgroup size the autotuner tries, and the
+FP32-storage population error against FP64 over 1000 fs at depth 4 (20,475 complex64
+24 x 24 ADOs)."""
 import numpy as np
 import pytest
 
@@ -106,3 +108,44 @@ def test_wide_fp32_population_error_1000fs(oracle_mod):
     err = np.max(np.abs(pops[FP32] - pops[heom_amd.HEOM_PRECISION_FP64]))
     print(f"24 sites, depth 4: FP32-storage max population error over 1000 fs {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("N", [9, 12, 16, 32])
+def test_wide_state_counts_match_oracle(oracle_mod, N):
+    """RHS and 5 RK4 steps for N = 9..32 (one bath mode per site, depth 2): up to N = 17
+    an 8-ADO FP64 tile fits the 48 KB LDS instance of the kernel, N = 32 runs the 144 KB
+    one."""
+    from test_gpu_parity import make_instance
+    inst = make_instance(N, N, 1, 2, [[i] for i in range(N)], seed=N)
+    y = random_state(inst, N)
+    ref_rhs = inst.rhs(y)
+    ref = inst.propagate(y.copy(), 1e-15, 5, 5)
+    with heom_amd.Handle.from_instance(inst) as h:
+        assert h.stage_kernel() == "wide"
+        h.set_state(y)
+        got_rhs = h.rhs()
+        tops = h.propagate(1e-15, 5, 5)
+    e1, e2 = max_abs_rel(got_rhs, ref_rhs), max_abs_rel(tops, ref)
+    print(f"wide N={N} ({inst.matrices} ADOs): rhs {e1:.1e}, 5 steps {e2:.1e}", flush=True)
+    assert e1 < 1e-12 and e2 < 1e-10
+
+
+@pytest.mark.parametrize("prec", [heom_amd.HEOM_PRECISION_FP64, FP32])
+def test_wide_every_tile_size_after_autotune(oracle_mod, prec):
+    """The autotuner times 8-, 4-, 2- and 1-ADO workgroups on the 24-site hierarchy (both
+    LDS instances) and restores the state; the tuned handle then matches an untuned one."""
+    _, inst = instance("excitonic24_d4", depth=2)
+    y = random_state(inst, 9)
+    with heom_amd.Handle.from_instance(inst, precision=prec) as h:
+        h.set_state(y)
+        res = h.autotune(steps=1)
+        assert {r["ados_per_workgroup"] for r in res} >= {8, 4, 2, 1}
+        assert np.array_equal(h.get_state(), y) or prec == FP32
+        h.set_state(y)
+        h.step_rk4(1e-15, 3)
+        tuned = h.get_state()
+    with heom_amd.Handle.from_instance(inst, precision=prec) as h:
+        h.set_state(y)
+        h.step_rk4(1e-15, 3)
+        plain = h.get_state()
+    assert np.array_equal(tuned, plain)
