@@ -33,8 +33,18 @@ def test_wide_rhs_fp64_matches_oracle(oracle_mod, depth):
     assert err < 1e-12
 
 
-def test_wide_rhs_fp32(oracle_mod):
-    _, inst = instance("excitonic24_d4", depth=2)
+def fp32_case(N):
+    if N == 24:
+        return instance("excitonic24_d4", depth=2)[1]
+    from test_gpu_parity import make_instance
+    return make_instance(N, N, 1, 2, [[i] for i in range(N)], seed=N)
+
+
+@pytest.mark.parametrize("N", [12, 24, 32])
+def test_wide_rhs_fp32(oracle_mod, N):
+    """FP32 storage: the RHS against the oracle on the FP32-rounded state (tier-0/1 ADOs
+    kept in FP64), for both LDS instances (N = 12 and 24: 48 KB, N = 32: 144 KB)."""
+    inst = fp32_case(N)
     y = random_state(inst, 7)
     t = tier01(inst)
     ys = as_complex(y.astype(np.float32).astype(np.float64), inst)
@@ -44,7 +54,7 @@ def test_wide_rhs_fp32(oracle_mod):
         h.set_state(y)
         got = h.rhs()
     err = max_abs_rel(got, ref)
-    print(f"N=24 FP32 rhs max rel deviation {err:.2e}", flush=True)
+    print(f"N={N} FP32 rhs max rel deviation {err:.2e}", flush=True)
     assert err < 1e-5
     a, b = as_complex(got, inst), as_complex(ref, inst)
     assert np.max(np.abs(a[t] - b[t])) / np.max(np.abs(b[t])) < 1e-12
