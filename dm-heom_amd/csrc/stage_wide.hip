@@ -65,7 +65,8 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
     C* sig = lds + mat;  // [T][N][NP] the tile, then commutator + same term
 
     const int64_t mp = a.mp;
-    const int64_t q0 = (int64_t)blockIdx.x * T;
+    // XCD-aware order: runs of 16 consecutive workgroups (16 T ADOs) on one XCD
+    const int64_t q0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x, 16) * T;
     const C* hs = hs_all + (a.tile_inst ? (int64_t)a.tile_inst[q0 / kTile] * nel : 0);
     for (int idx = threadIdx.x; idx < nel; idx += blockDim.x)
         hl[(idx / N) * NP + idx % N] = hs[idx];
