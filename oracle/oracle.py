@@ -930,6 +930,11 @@ def dipole_matrices(cfg: Config, tensor_index: int):
     return dm, dp
 
 
+def rotating_frame_phase(e0: float, times: np.ndarray) -> np.ndarray:
+    """exp(-i E0 t / hbar): the phase the excited block loses when E0 is taken out of it."""
+    return np.exp(-1j * e0 * times / HBAR)
+
+
 def linear_absorption(cfg: Config, nsteps: int | None = None, observe_steps: int | None = None,
                       rotating_frame: float = 0.0):
     """Averaged dipole correlation Tr(D- sigma_0(t)) over the tensor components.
@@ -956,6 +961,6 @@ def linear_absorption(cfg: Config, nsteps: int | None = None, observe_steps: int
         tops = inst.propagate(y, cfg.step_size, nsteps, observe_steps)
         tr = np.array([trace(dm, tops[r]) for r in range(tops.shape[0])])
         if rotating_frame != 0.0:
-            tr = tr * np.exp(-1j * rotating_frame * times / HBAR)
+            tr = tr * rotating_frame_phase(rotating_frame, times)
         acc += tr
     return times, acc / ncomp
