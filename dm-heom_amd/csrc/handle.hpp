@@ -141,6 +141,7 @@ struct heom_handle {
     DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
     DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded
     DevBuf<int32_t> q2u, tile_inst;
+    DevBuf<int32_t> tile_plus;              // [ntiles] 1 if an ADO of the tile has a plus edge
     DevBuf<int64_t> q_top;                  // [count] internal index of each top ADO
     DevBuf<uint8_t> mask;
     DevBuf<double2> pfss, hs, gam, mx, small;  // FP64 constants

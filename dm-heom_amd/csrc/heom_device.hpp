@@ -2,7 +2,7 @@
 // the HIP translation units of libheom_hip.so.
 //
 // Device layout of one hierarchy (FP64; FP32 storage halves the element size):
-//   * ADOs are renumbered into lexicographic order of their tuples (mode 0 most
+//   * ADOs are renumbered into (nearly) lexicographic order of their tuples (mode 0 most
 //     significant).  Neighbours n +- e_m then sit close in index: at 735k ADOs four
 //     fifths of all neighbour reads land within 30k ADOs, against one in twenty in the
 //     reference's layer order, so most of them are served from L2 / MALL.
@@ -17,6 +17,10 @@
 //     row W serves the padding entries of the per-element term lists, and row W+1 lets
 //     the pipelined kernel stage rows in pairs.  Packing limits M to 2^26 - 1
 //     ADOs and n_m to 62.
+//
+//   * Within that order the ADOs are grouped into tiles of 64 that either all have or
+//     all lack plus neighbours (the deepest tier has none); a stage kernel skips the
+//     plus gathers of a tile whose flag says it has none (tile_plus).
 //
 //   * A batch of hierarchies with the same states and baths is one concatenated
 //     hierarchy: instance k owns the tiles from its tile offset on (its ADOs padded to
@@ -98,6 +102,7 @@ struct StageArgsT {
     const int2* __restrict__ nb;     // [W+2][Mp] packed neighbour table
     const c2<R>* __restrict__ pfss;  // [Mp] pf_same_sum
     const int32_t* __restrict__ tile_inst;  // [ntiles] instance of each tile, or null (one instance)
+    const int32_t* __restrict__ tile_plus;  // [ntiles] 0 if no ADO of the tile has a plus edge, or null
     int64_t mp;
     int32_t ntiles;       // Mp / 64
     int32_t zero_slot;    // index of the zero slot (M)
@@ -135,6 +140,14 @@ __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c)
     c.y = fmaf(a.x, b.y, c.y);
     c.y = fmaf(a.y, b.x, c.y);
     return c;
+}
+
+// false if no ADO of `tile` has a plus neighbour: its plus gathers would all read the
+// zero slot, and the kernels leave them out (a wave-uniform branch per tile)
+template <typename R>
+__device__ __forceinline__ bool tile_has_plus(const StageArgsT<R>& a, int tile)
+{
+    return a.tile_plus == nullptr || a.tile_plus[tile] != 0;
 }
 
 // H / hbar of the instance a tile belongs to
@@ -269,7 +282,7 @@ void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const 
 // ---- large systems (stage_wide.hip): 8 < N <= 32, one wavefront per ADO ---------------
 
 bool wide_supported(int N, int complex_bytes);
-// the largest number of ADOs per workgroup (a power of two <= 16) whose tiles fit in LDS
+// the largest number of ADOs per workgroup (a power of two <= 8) whose tiles fit in LDS
 int wide_max_ados(int N, int complex_bytes);
 // ados: ADOs per workgroup, 0 = wide_max_ados
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,

@@ -397,6 +397,7 @@ StageArgsT<R> stage_args(heom_handle* h, const void* yin, const void* y0, void* 
     a.nb = h->nb.p;
     a.pfss = pfss;
     a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
+    a.tile_plus = h->tile_plus.p;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
     a.zero_slot = h->zero_slot;
@@ -419,6 +420,7 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     a.nb = h->nb.p;
     a.pfss = pfss;
     a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
+    a.tile_plus = h->tile_plus.p;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
     a.zero_slot = h->zero_slot;
@@ -575,6 +577,41 @@ void refold_tables(heom_handle* h)
     hipLaunchKernelGGL(fold_mask, dim3(blocks_for(count, 256)), dim3(256), 0, h->stream, h->nb_raw.p, h->nb.p,
                        h->mask.p, h->q2u.p, count, h->zero_slot);
     HIPC(hipGetLastError());
+}
+
+// Tile-homogeneous ADO order.  An ADO without plus neighbours (the deepest tier: two
+// thirds of the ADOs of the 8-site, 16-mode, depth-8 hierarchy) needs no plus gathers,
+// and a stage kernel skips them for a whole tile when none of its 64 ADOs has one.
+// `seq` lists one instance's ADOs in locality order (lexicographic); they are dealt
+// into two queues by class, and a queue that reaches 64 ADOs becomes the next tile.
+// So every tile but the last one or two holds one class, and the tiles still follow
+// `seq`: an ADO moves by about two tiles of the other class at most.
+std::vector<int32_t> group_tiles(const std::vector<int32_t>& seq, const std::vector<uint8_t>& has_plus)
+{
+    std::vector<int32_t> out, qd, qs;
+    out.reserve(seq.size());
+    for (const int32_t u : seq) {
+        std::vector<int32_t>& q = has_plus[(size_t)u] ? qs : qd;
+        q.push_back(u);
+        if ((int)q.size() == kTile) {
+            out.insert(out.end(), q.begin(), q.end());
+            q.clear();
+        }
+    }
+    out.insert(out.end(), qs.begin(), qs.end());
+    out.insert(out.end(), qd.begin(), qd.end());
+    return out;
+}
+
+// [ntiles] 1 if some ADO of the tile has a plus neighbour in the unmasked table, else 0
+void build_tile_plus(heom_handle* h, const std::vector<int2>& nb)
+{
+    std::vector<int32_t> f((size_t)h->ntiles, 0);
+    for (int m = 0; m < h->W; ++m)
+        for (int64_t q = 0; q < h->Mp; ++q)
+            if ((int32_t)((uint32_t)nb[(size_t)m * h->Mp + q].x >> 6) != h->zero_slot)
+                f[(size_t)(q / kTile)] = 1;
+    upload(h->tile_plus, f, h->stream);
 }
 
 // H [count][N*N*2] in J -> H / hbar on the device
@@ -925,23 +962,32 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
             return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W, T + (int64_t)b * W,
                                                 T + (int64_t)b * W + W);
         });
-        std::vector<int32_t>& u2q = u2q_all[k];
-        u2q.assign(M, 0);
         for (int64_t q = 0; q < M; ++q) {
-            u2q[order[q]] = (int32_t)q;
-            q2u[qb + q] = (int32_t)(h->inst_u[k] + order[q]);
             // the sorted tuples of heom_get_ados: lexicographic position -> uid
             const int64_t lx = h->inst_u[k] + q;
             for (int m = 0; m < W; ++m)
                 h->lex_tuples[(size_t)lx * W + m] = (uint8_t)T[(int64_t)order[q] * W + m];
             h->lex_uid[(size_t)lx] = (int32_t)(h->inst_u[k] + order[q]);
-            h->u2q_host[(size_t)(h->inst_u[k] + order[q])] = (int32_t)(qb + q);
+        }
+        // slots: the lexicographic order grouped into tiles with and without plus edges
+        std::vector<uint8_t> has_plus(M, 0);
+        for (int64_t u = 0; u < M; ++u)
+            for (int m = 0; m < W; ++m)
+                if (d[k].ado_plus[u * W + m] >= 0)
+                    has_plus[u] = 1;
+        const std::vector<int32_t> slots = group_tiles(order, has_plus);
+        std::vector<int32_t>& u2q = u2q_all[k];
+        u2q.assign(M, 0);
+        for (int64_t q = 0; q < M; ++q) {
+            u2q[slots[q]] = (int32_t)q;
+            q2u[qb + q] = (int32_t)(h->inst_u[k] + slots[q]);
+            h->u2q_host[(size_t)(h->inst_u[k] + slots[q])] = (int32_t)(qb + q);
         }
         qtop[k] = qb + u2q[0];
         for (int64_t t = qb / kTile; t < (qb + M + kTile - 1) / kTile; ++t)
             tinst[t] = k;
         for (int64_t q = 0; q < M; ++q) {
-            const int64_t u = order[q];
+            const int64_t u = slots[q];
             for (int m = 0; m < W; ++m) {
                 const int32_t p = d[k].ado_plus[u * W + m];
                 const int32_t mn = d[k].ado_minus[u * W + m];
@@ -956,6 +1002,7 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     }
     upload(h->nb_raw, nb, h->stream);
     upload(h->nb, nb, h->stream);
+    build_tile_plus(h, nb);
     upload(h->pfss, pfss, h->stream);
     upload(h->q2u, q2u, h->stream);
     upload(h->tile_inst, tinst, h->stream);
@@ -1085,6 +1132,53 @@ void mmult_left_planes(heom_handle* h)
     default: hipLaunchKernelGGL((mmult_left_kernel<C, 8>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
     }
     HIPC(hipGetLastError());
+}
+
+// One configuration of the stage kernels (the fields of heom_tune_result).
+struct StageConfig {
+    int kind, chunk, wgs, ados;
+};
+
+// every configuration the autotuner times for this handle: for N <= 8 the tile kernel
+// and, where it fits, the pipelined kernel (one workgroup per CU), each with 1, 2, 4, ...,
+// 64 tiles per XCD chunk and the default chunk; for N > 8 the wide kernel with every
+// power-of-two tile that fits
+std::vector<StageConfig> stage_candidates(heom_handle* h)
+{
+    std::vector<StageConfig> cands;
+    const int nt = h->ntiles;
+    auto chunks = [&](int dflt) {
+        std::vector<int> v{1, 2, 4, 8, 16, 32, 64, dflt};
+        std::vector<int> out;
+        for (int c : v)
+            if (c >= 1 && c <= nt && std::find(out.begin(), out.end(), c) == out.end())
+                out.push_back(c);
+        return out;
+    };
+    if (h->N > kMaxStates) {
+        for (int T = wide_max_ados(h->N, (int)h->cbytes()); T >= 1; T /= 2)
+            cands.push_back({HEOM_STAGE_WIDE, 1, 0, T});
+    } else {
+        for (int c : chunks(std::max(1, nt / kNumXcd)))
+            cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
+        if (pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes())) {
+            int cus = 0;
+            HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+            const int wgs = (std::max(1, std::min(cus, nt)) + kNumXcd - 1) / kNumXcd * kNumXcd;
+            for (int c : chunks(wgs / kNumXcd))
+                cands.push_back({HEOM_STAGE_PIPELINED, c, wgs, kTile});
+        }
+    }
+    return cands;
+}
+
+void apply_stage_config(heom_handle* h, const StageConfig& c)
+{
+    h->pipelined = c.kind == HEOM_STAGE_PIPELINED;
+    h->pipe_wgs = c.kind == HEOM_STAGE_PIPELINED ? c.wgs : h->pipe_wgs;
+    h->xcd_chunk = c.chunk;
+    h->wide_ados = c.kind == HEOM_STAGE_WIDE ? c.ados : 0;
+    drop_graph(h);
 }
 
 }  // namespace heom_rt
@@ -1315,10 +1409,16 @@ int heom_get_ados(heom_handle* h, int32_t instance, const int32_t* tuples, int64
             std::vector<double2> sh((size_t)h->S * nel);
             HIPC(hipMemcpyAsync(sh.data(), h->y64.p, sh.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
             HIPC(hipStreamSynchronize(h->stream));
+            // uid -> tier-0/1 index through a sorted copy: O((S + n) log S)
+            std::vector<std::pair<int64_t, int>> by_uid((size_t)h->S);
             for (int sp = 0; sp < h->S; ++sp)
-                for (int64_t a = 0; a < n; ++a)
-                    if (uid[(size_t)a] == h->sp_uid[(size_t)sp])
-                        std::memcpy(out + (size_t)a * nel * 2, &sh[(size_t)sp * nel], (size_t)nel * sizeof(double2));
+                by_uid[(size_t)sp] = {h->sp_uid[(size_t)sp], sp};
+            std::sort(by_uid.begin(), by_uid.end());
+            for (int64_t a = 0; a < n; ++a) {
+                const auto it = std::lower_bound(by_uid.begin(), by_uid.end(), std::make_pair(uid[(size_t)a], -1));
+                if (it != by_uid.end() && it->first == uid[(size_t)a])
+                    std::memcpy(out + (size_t)a * nel * 2, &sh[(size_t)it->second * nel], (size_t)nel * sizeof(double2));
+            }
         }
     });
 }
@@ -1691,13 +1791,23 @@ int heom_compact(heom_handle* h, int64_t* active)
         std::vector<int32_t> o2n(Mp, -1);
         std::vector<int64_t> new_q(h->count), new_n(h->count);
         int64_t q0 = 0, kept = 0;
+        auto kept_slot = [&](int64_t q) { return q != h->zero_slot && q2u[q] >= 0 && mask[q2u[q]] != 0; };
+        std::vector<uint8_t> has_plus(Mp, 0);
+        for (int m = 0; m < W; ++m)
+            for (int64_t q = 0; q < Mp; ++q)
+                if (kept_slot((int32_t)((uint32_t)nb[(size_t)m * Mp + q].x >> 6)))
+                    has_plus[q] = 1;
         for (int k = 0; k < h->count; ++k) {
             new_q[k] = q0;
             const int64_t lo = h->inst_q[k], hi = h->inst_q[k] + h->inst_n[k];
-            int64_t n = 0;
+            std::vector<int32_t> seq;
             for (int64_t q = lo; q < hi; ++q)
-                if (q2u[q] >= 0 && mask[q2u[q]] != 0)
-                    o2n[q] = (int32_t)(q0 + n++);
+                if (kept_slot(q))
+                    seq.push_back((int32_t)q);
+            const int64_t n = (int64_t)seq.size();
+            const std::vector<int32_t> grouped = group_tiles(seq, has_plus);
+            for (int64_t r = 0; r < n; ++r)
+                o2n[grouped[r]] = (int32_t)(q0 + r);
             if (o2n[qtop[k]] < 0)
                 raise(HEOM_E_INVALID, "the top ADO of every instance must stay unmasked");
             kept += n;
@@ -1748,7 +1858,6 @@ int heom_compact(heom_handle* h, int64_t* active)
         }
         upload(h->nb_raw, nb_n, h->stream);
         upload(h->nb, nb_n, h->stream);
-        upload(h->pfss, pfss_n, h->stream);
         upload(h->q2u, q2u_n, h->stream);
         upload(h->tile_inst, tinst, h->stream);
         upload(h->q_top, qtop, h->stream);
@@ -1756,6 +1865,8 @@ int heom_compact(heom_handle* h, int64_t* active)
         h->ntiles = ntiles;
         h->zero_slot = zero;
         h->active = kept;
+        build_tile_plus(h, nb_n);
+        upload(h->pfss, pfss_n, h->stream);
         h->inst_q = new_q;
         h->inst_n = new_n;
         std::fill(h->u2q_host.begin(), h->u2q_host.end(), -1);
@@ -1786,33 +1897,7 @@ int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int3
             raise(HEOM_E_INVALID, "steps must be >= 1");
         if (capacity > 0 && !results)
             raise(HEOM_E_INVALID, "results is NULL");
-        struct Cand {
-            int kind, chunk, wgs, ados;
-        };
-        std::vector<Cand> cands;
-        const int nt = h->ntiles;
-        auto chunks = [&](int dflt) {
-            std::vector<int> v{1, 2, 4, 8, 16, 32, 64, dflt};
-            std::vector<int> out;
-            for (int c : v)
-                if (c >= 1 && c <= nt && std::find(out.begin(), out.end(), c) == out.end())
-                    out.push_back(c);
-            return out;
-        };
-        if (h->N > kMaxStates) {
-            for (int T = wide_max_ados(h->N, (int)h->cbytes()); T >= 1; T /= 2)
-                cands.push_back({HEOM_STAGE_WIDE, 1, 0, T});
-        } else {
-            for (int c : chunks(std::max(1, nt / kNumXcd)))
-                cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
-            if (pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes())) {
-                int cus = 0;
-                HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-                const int wgs = (std::max(1, std::min(cus, nt)) + kNumXcd - 1) / kNumXcd * kNumXcd;
-                for (int c : chunks(wgs / kNumXcd))
-                    cands.push_back({HEOM_STAGE_PIPELINED, c, wgs, kTile});
-            }
-        }
+        const std::vector<StageConfig> cands = stage_candidates(h);
         // the state (and with FP32 storage its FP64 tier shadow) is restored after each trial
         DevBuf<unsigned char> save, save64;
         save.alloc(h->y.bytes());
@@ -1828,13 +1913,7 @@ int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int3
         HIPC(hipEventCreate(&e1));
         std::vector<double> ms(cands.size());
         const double dt = 1e-15;
-        auto apply = [&](const Cand& c) {
-            h->pipelined = c.kind == HEOM_STAGE_PIPELINED;
-            h->pipe_wgs = c.kind == HEOM_STAGE_PIPELINED ? c.wgs : h->pipe_wgs;
-            h->xcd_chunk = c.chunk;
-            h->wide_ados = c.kind == HEOM_STAGE_WIDE ? c.ados : 0;
-            drop_graph(h);
-        };
+        auto apply = [&](const StageConfig& c) { apply_stage_config(h, c); };
         try {
             for (size_t k = 0; k < cands.size(); ++k) {
                 apply(cands[k]);
@@ -1872,6 +1951,28 @@ int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int3
             results[k].chosen = k == best ? 1 : 0;
             results[k].ms_per_step = ms[k];
         }
+    });
+}
+
+int heom_set_stage_config(heom_handle* h, const heom_tune_result* config)
+{
+    return guard([&] {
+        need(h);
+        reject_tss(h, "heom_set_stage_config");
+        if (!config)
+            raise(HEOM_E_INVALID, "config is NULL");
+        const StageConfig c{config->kernel, config->xcd_chunk, config->workgroups, config->ados_per_workgroup};
+        bool ok = false;
+        for (const StageConfig& v : stage_candidates(h))
+            if (v.kind == c.kind && v.ados == c.ados && (c.kind != HEOM_STAGE_PIPELINED || v.wgs == c.wgs))
+                ok = true;
+        if (!ok)
+            raise(HEOM_E_INVALID, "stage configuration (kernel " + std::to_string(c.kind) + ", " +
+                                      std::to_string(c.ados) + " ADOs per workgroup, " + std::to_string(c.wgs) +
+                                      " workgroups) is not valid for this handle");
+        if (c.chunk < 1 || c.chunk > h->ntiles)
+            raise(HEOM_E_INVALID, "xcd_chunk must lie in [1, " + std::to_string(h->ntiles) + "]");
+        apply_stage_config(h, c);
     });
 }
 

@@ -38,22 +38,21 @@ namespace heom_hip {
 
 namespace {
 
-template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ>
-__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
-                                                                  const c2<R>* __restrict__ gam,
-                                                                  const TermT<R>* __restrict__ terms)
+// One tile; PLUS = false leaves out the plus gathers (no ADO of the tile has a plus
+// neighbour, so they would all read the zero slot).
+template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ, bool PLUS>
+__device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* __restrict__ hs_all,
+                                           const c2<R>* __restrict__ gam, const TermT<R>* __restrict__ terms,
+                                           int tile, c2<R>* sig)
 {
     using C = c2<R>;
     constexpr int CW = kStageColumns;
     constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
-    __shared__ C sig[N * N * kTile];
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = w / NJW;
     const int jb = (w % NJW) * CW;
-    const int t0 = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
-    const int tile = a.reverse ? a.ntiles - 1 - t0 : t0;
     const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
@@ -112,7 +111,8 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
         C xp[NT], xm[NT];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            xp[k] = plane[(uint32_t)nbv[k].x >> 6];
+            if constexpr (PLUS)
+                xp[k] = plane[(uint32_t)nbv[k].x >> 6];
             xm[k] = plane[(uint32_t)nbv[k].y];
         }
 
@@ -140,15 +140,19 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
             const int v = nbv[k].x & 63;
             if (ADJ) {
                 const R cp = t.plus_c * lane_value(sq_lane, v);
-                const R sp = lane_value(sq_lane, v + 1);
                 f.x = fma(-cp, xm[k].y, f.x);
                 f.y = fma(cp, xm[k].x, f.y);
-                f = cfma(t.minus_c, mk2<R>(sp * xp[k].x, sp * xp[k].y), f);
+                if constexpr (PLUS) {
+                    const R sp = lane_value(sq_lane, v + 1);
+                    f = cfma(t.minus_c, mk2<R>(sp * xp[k].x, sp * xp[k].y), f);
+                }
             } else {
-                const R cp = t.plus_c * lane_value(sq_lane, v + 1);
+                if constexpr (PLUS) {
+                    const R cp = t.plus_c * lane_value(sq_lane, v + 1);
+                    f.x = fma(-cp, xp[k].y, f.x);
+                    f.y = fma(cp, xp[k].x, f.y);
+                }
                 const R sm = lane_value(sq_lane, v);
-                f.x = fma(-cp, xp[k].y, f.x);
-                f.y = fma(cp, xp[k].x, f.y);
                 f = cfma(t.minus_c, mk2<R>(sm * xm[k].x, sm * xm[k].y), f);
             }
         }
@@ -161,6 +165,20 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a
             stage_epilogue<STAGE, R>(a, (int64_t)e * mp + q, s[c], f);
         }
     }
+}
+
+template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ>
+__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
+                                                                  const c2<R>* __restrict__ gam,
+                                                                  const TermT<R>* __restrict__ terms)
+{
+    __shared__ c2<R> sig[N * N * kTile];
+    const int t0 = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
+    const int tile = a.reverse ? a.ntiles - 1 - t0 : t0;
+    if (tile_has_plus(a, tile))
+        stage_tile<R, N, NT, STAGE, REALH, ADJ, true>(a, hs_all, gam, terms, tile, sig);
+    else
+        stage_tile<R, N, NT, STAGE, REALH, ADJ, false>(a, hs_all, gam, terms, tile, sig);
 }
 
 template <typename R, int N, int NT, bool REALH, bool ADJ>

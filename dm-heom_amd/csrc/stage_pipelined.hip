@@ -94,7 +94,7 @@ __device__ __forceinline__ void prefetch_tile(const StageArgsT<R>& a, int tile, 
 // after the first column's gathers are in flight: the previous tile's stores and the
 // next tile's prefetch go out there, behind those gathers, so that waiting for the
 // gathers does not mean waiting for the prefetch (loads complete in issue order).
-template <typename R, int N, int NT, int STAGE, bool REALH, typename Issue>
+template <typename R, int N, int NT, int STAGE, bool REALH, bool PLUS, typename Issue>
 __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>* __restrict__ hs_all,
                                              const c2<R>* __restrict__ gam, const TermT<R>* __restrict__ terms,
                                              int tile, int i, int jb, int lane, R sq_lane, const c2<R>* sig,
@@ -132,7 +132,7 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
     // the per-column arrays in registers.
     auto col = [&](int c) { return min(jb + c, N - 1); };
     // Neighbour gathers as raw buffer loads over the stage input: an edge to the zero
-    // slot (missing or masked neighbour, two thirds of the lanes at cfg1) gets an offset
+    // slot (missing or masked neighbour; most minus lanes of the deepest tier) gets an offset
     // past the buffer's size, which returns zero without a memory request.
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)a.yin, (short)0, (int)(N * N * mp * (int64_t)sizeof(C)), 0x00020000);
@@ -156,7 +156,8 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
             L[c].nbv[k] = tab[te[k].mode * kTile + lane];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            L[c].xp[k] = gather(off_e, (uint32_t)L[c].nbv[k].x >> 6);
+            if constexpr (PLUS)
+                L[c].xp[k] = gather(off_e, (uint32_t)L[c].nbv[k].x >> 6);
             L[c].xm[k] = gather(off_e, (uint32_t)L[c].nbv[k].y);
         }
     };
@@ -205,10 +206,12 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int v = L[c].nbv[k].x & 63;
-            const R cp = te[k].plus_c * lane_value(sq_lane, v + 1);
+            if constexpr (PLUS) {
+                const R cp = te[k].plus_c * lane_value(sq_lane, v + 1);
+                f.x = fma(-cp, L[c].xp[k].y, f.x);
+                f.y = fma(cp, L[c].xp[k].x, f.y);
+            }
             const R sm = lane_value(sq_lane, v);
-            f.x = fma(-cp, L[c].xp[k].y, f.x);
-            f.y = fma(cp, L[c].xp[k].x, f.y);
             f = cfma(te[k].minus_c, mk2<R>(sm * L[c].xm[k].x, sm * L[c].xm[k].y), f);
         }
         stage_values<STAGE, R>(a.c1, s, f, ep, r0[c], r1[c]);
@@ -264,18 +267,28 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgsT
     prefetch_tile<R, N>(a, phys(t), w, i, jb, lane, sigA, tabA);
     __syncthreads();  // waits for the loads of this wave, then for every wave
 
+    // a tile whose ADOs have no plus neighbours (tile_plus) runs the variant without plus
+    // gathers; the branch is uniform over the workgroup
+    auto compute = [&](int tile, const C* sig, const int2* tab, auto&& issue) {
+        if (tile_has_plus(a, tile))
+            compute_tile<R, N, NT, STAGE, REALH, true>(a, hs, gam, terms, tile, i, jb, lane, sq_lane, sig, tab, r0, r1,
+                                                       issue);
+        else
+            compute_tile<R, N, NT, STAGE, REALH, false>(a, hs, gam, terms, tile, i, jb, lane, sq_lane, sig, tab, r0,
+                                                        r1, issue);
+    };
+
     int prev = -1;
     for (;;) {
         // tile t is in A; the next one goes to B
         j += gx;
         int tn = chunk_tile(x, j, chunk);
-        compute_tile<R, N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigA, tabA, r0, r1,
-                                             [&] {
-                                                 if (prev >= 0)
-                                                     store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
-                                                 if (tn < nt)
-                                                     prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigB, tabB);
-                                             });
+        compute(phys(t), sigA, tabA, [&] {
+            if (prev >= 0)
+                store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+            if (tn < nt)
+                prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigB, tabB);
+        });
         prev = t;
         t = tn;
         __syncthreads();  // B is complete, and nobody reads A any more
@@ -285,12 +298,11 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage_pipe(StageArgsT
         // tile t is in B; the next one goes to A
         j += gx;
         tn = chunk_tile(x, j, chunk);
-        compute_tile<R, N, NT, STAGE, REALH>(a, hs, gam, terms, phys(t), i, jb, lane, sq_lane, sigB, tabB, r0, r1,
-                                             [&] {
-                                                 store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
-                                                 if (tn < nt)
-                                                     prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigA, tabA);
-                                             });
+        compute(phys(t), sigB, tabB, [&] {
+            store_tile<R, N, STAGE>(a, phys(prev), i, jb, lane, r0, r1);
+            if (tn < nt)
+                prefetch_tile<R, N>(a, phys(tn), w, i, jb, lane, sigA, tabA);
+        });
         prev = t;
         t = tn;
         __syncthreads();

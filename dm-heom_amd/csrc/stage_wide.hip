@@ -9,6 +9,8 @@
 // divides the 64-ADO instance granularity), and works in the three phases described
 // at the kernel: the commutator per ADO wavefront from LDS, everything that touches
 // global memory plane by plane across the tile's consecutive ADOs.
+#include <type_traits>
+
 #include "heom_device.hpp"
 
 namespace heom_hip {
@@ -138,58 +140,70 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
     }
     __syncthreads();
 
-    // phase C, two (ADO, element) pairs per thread and pass so that their loads overlap
-    const int total = T * nel, stride = blockDim.x;
-    for (int base = threadIdx.x; base < total; base += 2 * stride) {
-        int2 nbv[2][NT];
-        C ep[2], own[2], xp[2][NT], xm[2][NT];
-        int64_t o[2];
-        int e2[2], lo[2];
-        bool live[2];
+    // phase C, two (ADO, element) pairs per thread and pass so that their loads overlap;
+    // PLUS = false (no ADO of the 64-ADO tile that holds these T has a plus neighbour)
+    // leaves out the plus gathers
+    auto phase_c = [&](auto plus_tag) {
+        constexpr bool PLUS = decltype(plus_tag)::value;
+        const int total = T * nel, stride = blockDim.x;
+        for (int base = threadIdx.x; base < total; base += 2 * stride) {
+            int2 nbv[2][NT];
+            C ep[2], own[2], xp[2][NT], xm[2][NT];
+            int64_t o[2];
+            int e2[2], lo[2];
+            bool live[2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int idx = base + u * stride;
-            live[u] = idx < total;
-            const int id = live[u] ? idx : base;  // a dead second pair repeats the first's loads
-            const int ad = id % T, e = id / T;
-            e2[u] = e;
-            lo[u] = ad * mat + (e / N) * NP + e % N;
-            o[u] = (int64_t)e * mp + q0 + ad;
+            for (int u = 0; u < 2; ++u) {
+                const int idx = base + u * stride;
+                live[u] = idx < total;
+                const int id = live[u] ? idx : base;  // a dead second pair repeats the first's loads
+                const int ad = id % T, e = id / T;
+                e2[u] = e;
+                lo[u] = ad * mat + (e / N) * NP + e % N;
+                o[u] = (int64_t)e * mp + q0 + ad;
 #pragma unroll
-            for (int k = 0; k < NT; ++k)
-                nbv[u][k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q0 + ad];
-            ep[u] = stage_operand<STAGE, R>(a, o[u]);
-            own[u] = a.yin[o[u]];
-        }
+                for (int k = 0; k < NT; ++k)
+                    nbv[u][k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q0 + ad];
+                ep[u] = stage_operand<STAGE, R>(a, o[u]);
+                own[u] = a.yin[o[u]];
+            }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const C* plane = a.yin + (int64_t)e2[u] * mp;
+            for (int u = 0; u < 2; ++u) {
+                const C* plane = a.yin + (int64_t)e2[u] * mp;
 #pragma unroll
-            for (int k = 0; k < NT; ++k) {
-                xp[u][k] = plane[(uint32_t)nbv[u][k].x >> 6];
-                xm[u][k] = plane[(uint32_t)nbv[u][k].y];
+                for (int k = 0; k < NT; ++k) {
+                    if constexpr (PLUS)
+                        xp[u][k] = plane[(uint32_t)nbv[u][k].x >> 6];
+                    xm[u][k] = plane[(uint32_t)nbv[u][k].y];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (!live[u])
+                    continue;
+                C f = sig[lo[u]];
+#pragma unroll
+                for (int k = 0; k < NT; ++k) {
+                    const TermT<R> t = terms[e2[u] * NT + k];
+                    const int v = nbv[u][k].x & 63;
+                    if constexpr (PLUS) {
+                        const R cp = t.plus_c * sqrt((R)(v + 1));
+                        f.x = fma(-cp, xp[u][k].y, f.x);
+                        f.y = fma(cp, xp[u][k].x, f.y);
+                    }
+                    const R sm = sqrt((R)v);
+                    f = cfma(t.minus_c, mk2<R>(sm * xm[u][k].x, sm * xm[u][k].y), f);
+                }
+                C r0, r1 = mk2<R>(R(0), R(0));
+                stage_values<STAGE, R>(a.c1, own[u], f, ep[u], r0, r1);
+                stage_store<STAGE, R>(a, o[u], r0, r1);
             }
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            if (!live[u])
-                continue;
-            C f = sig[lo[u]];
-#pragma unroll
-            for (int k = 0; k < NT; ++k) {
-                const TermT<R> t = terms[e2[u] * NT + k];
-                const int v = nbv[u][k].x & 63;
-                const R cp = t.plus_c * sqrt((R)(v + 1));
-                const R sm = sqrt((R)v);
-                f.x = fma(-cp, xp[u][k].y, f.x);
-                f.y = fma(cp, xp[u][k].x, f.y);
-                f = cfma(t.minus_c, mk2<R>(sm * xm[u][k].x, sm * xm[u][k].y), f);
-            }
-            C r0, r1 = mk2<R>(R(0), R(0));
-            stage_values<STAGE, R>(a.c1, own[u], f, ep[u], r0, r1);
-            stage_store<STAGE, R>(a, o[u], r0, r1);
-        }
-    }
+    };
+    if (tile_has_plus(a, (int)(q0 / kTile)))
+        phase_c(std::true_type{});
+    else
+        phase_c(std::false_type{});
 }
 
 template <typename R, int B, int NT, bool REALH, int LDSB>

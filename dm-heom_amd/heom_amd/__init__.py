@@ -40,7 +40,7 @@ EXPORTED = [
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
     "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados", "heom_observe",
-    "heom_create_tss_batch",
+    "heom_create_tss_batch", "heom_set_stage_config",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -132,6 +132,7 @@ def load():
             "heom_rhs_adjoint": [vp, vp],
             "heom_autotune": [vp, ctypes.c_int32, ctypes.POINTER(TuneResult), ctypes.c_int32,
                               ctypes.POINTER(ctypes.c_int32)],
+            "heom_set_stage_config": [vp, ctypes.POINTER(TuneResult)],
             "heom_compact": [vp, ctypes.POINTER(i64)],
             "heom_propagate_timedep": [vp, dbl, i64, i64, vp, vp],
             "heom_create_tss": [ctypes.POINTER(Desc), vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
@@ -523,6 +524,14 @@ class Handle:
         return [{"kernel": STAGE_KINDS[r.kernel], "xcd_chunk": r.xcd_chunk, "workgroups": r.workgroups,
                  "ados_per_workgroup": r.ados_per_workgroup, "chosen": bool(r.chosen), "ms_per_step": r.ms_per_step}
                 for r in res[:min(n.value, cap)]]
+
+    def set_stage_config(self, config: dict) -> None:
+        """Apply one configuration as returned by autotune() (kernel, xcd_chunk,
+        workgroups, ados_per_workgroup) without timing anything."""
+        kinds = {v: k for k, v in STAGE_KINDS.items()}
+        r = TuneResult(kernel=kinds[config["kernel"]], xcd_chunk=int(config["xcd_chunk"]),
+                       workgroups=int(config["workgroups"]), ados_per_workgroup=int(config["ados_per_workgroup"]))
+        _check(load().heom_set_stage_config(self._h, ctypes.byref(r)))
 
     # adjoint
     def rhs_adjoint(self) -> np.ndarray:

@@ -254,6 +254,13 @@ typedef struct heom_tune_result {
 } heom_tune_result;
 int heom_autotune(heom_handle* handle, int32_t steps, heom_tune_result* results, int32_t capacity,
                   int32_t* count);
+/* Applies one configuration of the stage kernels without timing anything (no
+ * counterpart in the reference): a result of heom_autotune on a handle of the same
+ * shape, so a tuned configuration can be reused.  kernel, workgroups (pipelined kernel)
+ * and ados_per_workgroup must be a candidate heom_autotune would time for this handle;
+ * xcd_chunk lies in [1, tiles]; chosen and ms_per_step are ignored.  Results do not
+ * depend on the configuration. */
+int heom_set_stage_config(heom_handle* handle, const heom_tune_result* config);
 
 /* ---- adjoint: gradients of observables ------------------------------------------ */
 /* No counterpart in the reference, which has no differentiation; for fitting

@@ -148,9 +148,10 @@ def test_wide_every_tile_size_after_autotune(oracle_mod, prec):
     y = random_state(inst, 9)
     with heom_amd.Handle.from_instance(inst, precision=prec) as h:
         h.set_state(y)
+        before = h.get_state()  # FP32 storage rounds y
         res = h.autotune(steps=1)
         assert {r["ados_per_workgroup"] for r in res} >= {8, 4, 2, 1}
-        assert np.array_equal(h.get_state(), y) or prec == FP32
+        assert np.array_equal(h.get_state(), before)
         h.set_state(y)
         h.step_rk4(1e-15, 3)
         tuned = h.get_state()
@@ -159,3 +160,34 @@ def test_wide_every_tile_size_after_autotune(oracle_mod, prec):
         h.step_rk4(1e-15, 3)
         plain = h.get_state()
     assert np.array_equal(tuned, plain)
+
+
+@pytest.mark.parametrize("prec", [heom_amd.HEOM_PRECISION_FP64, FP32])
+def test_wide_forced_tile_sizes_are_bitwise_equal(prec):
+    """Every tile size the autotuner offers, forced with set_stage_config on the 24-site
+    hierarchy at depth 2 (325 ADOs in 384 slots: 1- and 2-ADO tiles give 384 and 192 workgroups, so
+    the XCD-aware workgroup order is active), gives bitwise the 8-ADO result: the tile
+    size changes only which workgroup computes an ADO."""
+    _, inst = instance("excitonic24_d4", depth=2)
+    y = random_state(inst, 11)
+    results = {}
+    with heom_amd.Handle.from_instance(inst, precision=prec) as h:
+        configs = h.autotune(steps=1)
+    for cfg in configs:
+        with heom_amd.Handle.from_instance(inst, precision=prec) as h:
+            h.set_stage_config(cfg)
+            h.set_state(y)
+            h.step_rk4(1e-15, 3)
+            results[cfg["ados_per_workgroup"]] = h.get_state()
+    assert set(results) >= {8, 4, 2, 1}
+    for T, got in results.items():
+        assert np.array_equal(got, results[8]), T
+
+
+def test_set_stage_config_refuses_invalid():
+    _, inst = instance("excitonic24_d4", depth=2)
+    with heom_amd.Handle.from_instance(inst) as h:
+        with pytest.raises(heom_amd.HeomError):
+            h.set_stage_config({"kernel": "wide", "xcd_chunk": 1, "workgroups": 0, "ados_per_workgroup": 3})
+        with pytest.raises(heom_amd.HeomError):
+            h.set_stage_config({"kernel": "pipelined", "xcd_chunk": 1, "workgroups": 256, "ados_per_workgroup": 64})
