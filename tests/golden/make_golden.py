@@ -14,8 +14,9 @@ system (3003 ADOs, 100 steps), the 8-site 16-mode system at depth 3 (969 ADOs, 5
 and the FMO linear-absorption example with its ground state (depth 2, 100 steps of the
 D+ |g><g| start).  Run from the repository root:  python tests/golden/make_golden.py
 
-``--full-size`` writes ``excitonic8_d8_full.npz`` instead: the whole 8-site, 16-mode,
-depth-8 workload (735,471 ADOs) for all 500 steps of its config, about 20 minutes of
+``--full-size`` writes ``full/excitonic8_d8_full.npz`` instead (kept apart from the small
+fixtures, which the oracle re-runs in the CPU suite): the whole 8-site, 16-mode,
+depth-8 workload (735,471 ADOs) for all 500 steps of its config, about 50 minutes of
 oracle time on 8 cores.  It holds
 
 * ``tops``         [51, 8, 8]     the top ADO every 10 steps (first = start)
@@ -107,7 +108,9 @@ def make_full(steps=500, observe_steps=10, stride=61):
     tier_max = np.array([norms[tier == d].max() for d in range(D + 1)])
     tier_count = np.array([int((tier == d).sum()) for d in range(D + 1)])
     checksum = float((norms * tuple_weights(T)).sum())
-    np.savez_compressed(os.path.join(OUT, "excitonic8_d8_full.npz"),
+    full = os.path.join(os.path.dirname(OUT), "full")
+    os.makedirs(full, exist_ok=True)
+    np.savez_compressed(os.path.join(full, "excitonic8_d8_full.npz"),
                         tops=tops, tuples=T[keep].astype(np.int32), ados=ados,
                         norm_tuples=T[sub].astype(np.int8), norms=norms[sub],
                         tier_norm_sum=tier_sum, tier_norm_max=tier_max, tier_count=tier_count,
