@@ -30,7 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dm-heom_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "rk4_stage_excitonic8_d8_traffic.json")
+# per-launch memory-side traffic of the stage kernels, measured with rocprofv3 counters
+# (profiles/*.md give the commands), one record per workload and storage precision
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "rk4_stage_traffic.json")
 
 
 def parse_args():
@@ -79,16 +81,32 @@ def copy_ceiling(gib: float = 1.0, reps: int = 5) -> float:
 
 
 def host_cpu() -> dict:
-    model = ""
+    """The host the CPU baseline ran on: lscpu's model name, vendor, family/model/stepping
+    and socket/core/thread counts, nproc, and OMP_NUM_THREADS as set in the environment."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Vendor ID": "vendor", "CPU family": "family", "Model": "model_number",
+                "Stepping": "stepping", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "CPU max MHz": "max_mhz"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return {"model": model, "nproc": os.cpu_count()}
+    if "model" not in info:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["model"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
 
 
 def cpu_baseline(cfg_path: str, target_s: float) -> dict:
@@ -465,10 +483,12 @@ def main():
     per_launch_s = ev / launches
     alg_bytes_per_launch = bytes_per_ado_step * M / 4
     achieved = alg_bytes_per_launch / per_launch_s / 1e9
-    traffic = None  # measured for the FP64 kernels (profiles/), not for FP32 storage
-    if not fp32 and os.path.exists(PROFILE_TRAFFIC):
+    traffic, traffic_rec = None, None
+    if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic_rec = json.load(f).get(f"{args.config}/{args.precision}")
+        if traffic_rec:
+            traffic = traffic_rec["l2_fabric_bytes_per_launch"]
 
     out = {
         "metric": "ADO\u00b7RK4-steps/s + % HBM peak; batched FP32 vs plain FP64 at \u22641e-6 pop err",
@@ -499,6 +519,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_note": None if traffic is None else (
+                "bytes per launch between the L2 and the memory fabric (rocprofv3: FETCH_SIZE x 2 + WRITE_SIZE; "
+                "every L2 read request is 128 B on gfx950, TCC_EA0_RDREQ x 128 B equals FETCH_SIZE x 2); it "
+                "includes hits in the 256 MB last-level cache (MALL), which no L2-side counter separates from "
+                f"DRAM (TCC_EA0_RDREQ_DRAM equals TCC_EA0_RDREQ); measured on build {traffic_rec.get('build')}, "
+                f"{traffic_rec.get('source')}"),
             "kernel": "rk4_stage (RHS + RK4 stage epilogue), 4 launches per step",
             "alg_bytes_per_launch": alg_bytes_per_launch,
             "avg_launch_ms": per_launch_s * 1e3,
