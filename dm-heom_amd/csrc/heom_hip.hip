@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <stdexcept>
@@ -28,6 +29,9 @@
 
 using namespace heom_hip;
 using namespace heom_rt;
+
+// size of the device scratch of heom_propagate_observe_noalloc (one array per GPU)
+constexpr size_t kObserveScratchBytes = HEOM_OBSERVE_SCRATCH_BYTES;
 
 thread_local std::string heom_rt::g_error;
 
@@ -1181,6 +1185,102 @@ void apply_stage_config(heom_handle* h, const StageConfig& c)
     drop_graph(h);
 }
 
+void check_observe_args(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps, const double* ops,
+                        double* pops, double* traces)
+{
+    if (!pops)
+        raise(HEOM_E_INVALID, "pops is NULL");
+    if (ops && !traces)
+        raise(HEOM_E_INVALID, "traces is NULL while ops is given");
+    if (nsteps < 0 || observe_steps < 1)
+        raise(HEOM_E_INVALID, "nsteps must be >= 0 and observe_steps >= 1");
+    if (!(dt > 0.0))
+        raise(HEOM_E_INVALID, "step size must be > 0");
+}
+
+// The propagation of heom_propagate_observe with caller-provided device buffers: opsd
+// [count][nel] (when ops is given), prec [rows][count][N], trec [rows][count].  With
+// graphs = false the stages are launched one by one even when step graphs are enabled,
+// so that nothing is captured or instantiated.
+void propagate_observe_into(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps, const double* ops,
+                            double* pops, double* traces, double2* opsd, double* prec, double2* trec, bool graphs)
+{
+    const int64_t rows = nsteps / observe_steps + 1;
+    if (ops)
+        HIPC(hipMemcpyAsync(opsd, ops, (size_t)h->count * h->nel * sizeof(double2), hipMemcpyHostToDevice,
+                            h->stream));
+    auto observe = [&](int64_t row) {
+        gather_tops(h, h->small.p);
+        hipLaunchKernelGGL(observe_tops, dim3(h->count), dim3(64), 0, h->stream, h->small.p, ops ? opsd : nullptr,
+                           prec + row * h->count * h->N, ops ? trec + row * h->count : nullptr, h->N);
+        HIPC(hipGetLastError());
+    };
+    resolve_timing(h);
+    HIPC(hipEventRecord(h->ev_start, h->stream));
+    observe(0);
+    int64_t row = 1, done = 0;
+    while (done < nsteps) {
+        const int64_t n = std::min(observe_steps, nsteps - done);
+        if (graphs) {
+            rk4_steps(h, dt, n);
+        } else {
+            reject_tss(h, "RK4 propagation");
+            for (int64_t s = 0; s < n; ++s)
+                rk4_step_launches(h, dt);
+            h->timing.rhs_launches += 4 * n;
+        }
+        done += n;
+        if (n == observe_steps)
+            observe(row++);
+    }
+    HIPC(hipEventRecord(h->ev_stop, h->stream));
+    h->timing_pending = true;
+    h->pending_steps = nsteps;
+    HIPC(hipMemcpyAsync(pops, prec, (size_t)(rows * h->count * h->N) * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+    if (ops)
+        HIPC(hipMemcpyAsync(traces, trec, (size_t)(rows * h->count) * sizeof(double2), hipMemcpyDeviceToHost,
+                            h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    resolve_timing(h);
+}
+
+}  // namespace heom_rt
+
+// Scratch of heom_propagate_observe_noalloc: a device array of fixed size, one copy per
+// GPU for the life of the process.  Its device address comes from hipGetSymbolAddress,
+// once per device with that device current; in host code the array's name is not a
+// device address.  Every call on a GPU shares that GPU's copy, so calls that overlap on
+// one GPU race; each call writes all of the scratch it reads.
+static __device__ __align__(256) unsigned char scratch[kObserveScratchBytes];
+
+namespace heom_rt {
+
+namespace {
+std::mutex g_scratch_mutex;
+std::vector<unsigned char*> g_scratch_ptr;  // per device, null until first use
+}  // namespace
+
+unsigned char* observe_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= device)
+        g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)device]) {
+        int cur = 0;
+        HIPC(hipGetDevice(&cur));
+        if (cur != device)
+            HIPC(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device)
+            HIPC(hipSetDevice(cur));
+        HIPC(e);
+        g_scratch_ptr[(size_t)device] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[(size_t)device];
+}
+
 }  // namespace heom_rt
 
 extern "C" {
@@ -1624,14 +1724,7 @@ int heom_propagate_observe(heom_handle* h, double dt, int64_t nsteps, int64_t ob
 {
     return guard([&] {
         need(h);
-        if (!pops)
-            raise(HEOM_E_INVALID, "pops is NULL");
-        if (ops && !traces)
-            raise(HEOM_E_INVALID, "traces is NULL while ops is given");
-        if (nsteps < 0 || observe_steps < 1)
-            raise(HEOM_E_INVALID, "nsteps must be >= 0 and observe_steps >= 1");
-        if (!(dt > 0.0))
-            raise(HEOM_E_INVALID, "step size must be > 0");
+        check_observe_args(h, dt, nsteps, observe_steps, ops, pops, traces);
         const int64_t rows = nsteps / observe_steps + 1;
         DevBuf<double> prec;
         DevBuf<double2> trec, opsd;
@@ -1639,34 +1732,35 @@ int heom_propagate_observe(heom_handle* h, double dt, int64_t nsteps, int64_t ob
         if (ops) {
             trec.alloc((size_t)(rows * h->count));
             opsd.alloc((size_t)h->count * h->nel);
-            HIPC(hipMemcpyAsync(opsd.p, ops, opsd.bytes(), hipMemcpyHostToDevice, h->stream));
         }
-        auto observe = [&](int64_t row) {
-            gather_tops(h, h->small.p);
-            hipLaunchKernelGGL(observe_tops, dim3(h->count), dim3(64), 0, h->stream, h->small.p,
-                               ops ? opsd.p : nullptr, prec.p + row * h->count * h->N,
-                               ops ? trec.p + row * h->count : nullptr, h->N);
-            HIPC(hipGetLastError());
-        };
-        resolve_timing(h);
-        HIPC(hipEventRecord(h->ev_start, h->stream));
-        observe(0);
-        int64_t row = 1, done = 0;
-        while (done < nsteps) {
-            const int64_t n = std::min(observe_steps, nsteps - done);
-            rk4_steps(h, dt, n);
-            done += n;
-            if (n == observe_steps)
-                observe(row++);
-        }
-        HIPC(hipEventRecord(h->ev_stop, h->stream));
-        h->timing_pending = true;
-        h->pending_steps = nsteps;
-        HIPC(hipMemcpyAsync(pops, prec.p, prec.bytes(), hipMemcpyDeviceToHost, h->stream));
-        if (ops)
-            HIPC(hipMemcpyAsync(traces, trec.p, trec.bytes(), hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
-        resolve_timing(h);
+        propagate_observe_into(h, dt, nsteps, observe_steps, ops, pops, traces, opsd.p, prec.p, trec.p, true);
+    });
+}
+
+int heom_propagate_observe_noalloc(heom_handle* h, double dt, int64_t nsteps, int64_t observe_steps,
+                                   const double* ops, double* pops, double* traces)
+{
+    return guard([&] {
+        need(h);
+        check_observe_args(h, dt, nsteps, observe_steps, ops, pops, traces);
+        const int64_t rows = nsteps / observe_steps + 1;
+        // scratch layout: ops [count][nel] double2, then the traces [rows][count] double2,
+        // then the populations [rows][count][N] double (each part 256-byte aligned)
+        auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+        const int64_t ops_b = ops ? up((int64_t)h->count * h->nel * (int64_t)sizeof(double2)) : 0;
+        const int64_t tr_b = ops ? up(rows * h->count * (int64_t)sizeof(double2)) : 0;
+        const int64_t po_b = up(rows * h->count * h->N * (int64_t)sizeof(double));
+        if (ops_b + tr_b + po_b > (int64_t)kObserveScratchBytes)
+            raise(HEOM_E_UNSUPPORTED,
+                  "heom_propagate_observe_noalloc: " + std::to_string(rows) + " observations of " +
+                      std::to_string(h->count) + " instances need " + std::to_string(ops_b + tr_b + po_b) +
+                      " bytes of scratch, more than its " + std::to_string(kObserveScratchBytes) +
+                      " (split the run into calls with fewer steps)");
+        unsigned char* base = observe_scratch(h->device);
+        double2* opsd = ops ? reinterpret_cast<double2*>(base) : nullptr;
+        double2* trec = ops ? reinterpret_cast<double2*>(base + ops_b) : nullptr;
+        double* prec = reinterpret_cast<double*>(base + ops_b + tr_b);
+        propagate_observe_into(h, dt, nsteps, observe_steps, ops, pops, traces, opsd, prec, trec, false);
     });
 }
 

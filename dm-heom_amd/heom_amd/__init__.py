@@ -40,7 +40,7 @@ EXPORTED = [
     "heom_rk4_step_bytes", "heom_stage_kernel", "heom_propagate_observe", "heom_use_graphs",
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
     "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados", "heom_observe",
-    "heom_create_tss_batch", "heom_set_stage_config",
+    "heom_create_tss_batch", "heom_set_stage_config", "heom_propagate_observe_noalloc",
 ]
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
@@ -126,6 +126,7 @@ def load():
             "heom_rk4_step_bytes": [vp, ctypes.POINTER(i64)],
             "heom_stage_kernel": [vp, ctypes.POINTER(ctypes.c_int)],
             "heom_propagate_observe": [vp, dbl, i64, i64, vp, vp, vp],
+            "heom_propagate_observe_noalloc": [vp, dbl, i64, i64, vp, vp, vp],
             "heom_use_graphs": [vp, ctypes.c_int],
             "heom_adjoint_forward": [vp, dbl, i64, i64, i64, vp, vp],
             "heom_adjoint_backward": [vp, vp, ctypes.POINTER(Gradients)],
@@ -464,16 +465,18 @@ class Handle:
         t = t[..., 0] + 1j * t[..., 1]
         return t[:, 0] if self.count == 1 else t
 
-    def propagate_observe(self, h: float, nsteps: int, observe_steps: int, ops=None):
+    def propagate_observe(self, h: float, nsteps: int, observe_steps: int, ops=None, noalloc: bool = False):
         """Populations (rows, count, N) and, with ops (an N x N operator, or one per
-        instance), traces Tr(ops sigma_0) (rows, count), reduced on the device."""
+        instance), traces Tr(ops sigma_0) (rows, count), reduced on the device.
+        noalloc=True calls heom_propagate_observe_noalloc (no device allocation; the
+        records must fit its fixed scratch)."""
         rows = nsteps // observe_steps + 1
         pops = np.empty(rows * self.count * self.states)
         traces = np.empty(rows * self.count * 2)
         o = self._per_instance(ops) if ops is not None else None
-        _check(load().heom_propagate_observe(self._h, float(h), int(nsteps), int(observe_steps),
-                                             o.ctypes.data if o is not None else None, pops.ctypes.data,
-                                             traces.ctypes.data if o is not None else None))
+        fn = load().heom_propagate_observe_noalloc if noalloc else load().heom_propagate_observe
+        _check(fn(self._h, float(h), int(nsteps), int(observe_steps), o.ctypes.data if o is not None else None,
+                  pops.ctypes.data, traces.ctypes.data if o is not None else None))
         tr = traces.reshape(rows, self.count, 2)
         return pops.reshape(rows, self.count, self.states), (tr[..., 0] + 1j * tr[..., 1] if o is not None else None)
 

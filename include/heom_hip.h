@@ -168,6 +168,20 @@ int heom_propagate(heom_handle* handle, double h, int64_t nsteps, int64_t observ
  * FP32 storage, on the FP64 top), one wavefront per instance. */
 int heom_propagate_observe(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
                            const double* ops, double* pops, double* traces);
+/* heom_propagate_observe without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code around HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The observation
+ * records and the operators live in a fixed device scratch array of
+ * HEOM_OBSERVE_SCRATCH_BYTES per GPU, held until the process ends; a call whose
+ * records do not fit (about rows * count * (8 N + 16) bytes plus 16 N^2 count for ops)
+ * fails with HEOM_E_UNSUPPORTED and changes nothing, so a long run is split into calls of
+ * fewer steps.  The stage kernels are launched one by one even when step graphs are
+ * enabled (no graph is captured or instantiated).  Results equal heom_propagate_observe
+ * bit for bit.  All calls on one GPU share its scratch: calls that overlap in time on
+ * one GPU (other threads, other handles) race and must be serialised by the caller. */
+#define HEOM_OBSERVE_SCRATCH_BYTES (8u << 20)
+int heom_propagate_observe_noalloc(heom_handle* handle, double h, int64_t nsteps, int64_t observe_steps,
+                                   const double* ops, double* pops, double* traces);
 /* The observations of the current state, as heom_propagate_observe records them: the
  * populations Re sigma_0[i][i] (matrix_diagonal_observer.hpp:28-34) into pops
  * [count][N] and, when ops [count][N*N*2] is given, Tr(ops_k sigma_0)
