@@ -487,7 +487,9 @@ def main():
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             traffic_rec = json.load(f).get(f"{args.config}/{args.precision}")
-        if traffic_rec:
+        kept = next(r for r in tuning if r["chosen"])
+        # the traffic of a stage launch depends on the tiles per XCD chunk the tuner kept
+        if traffic_rec and traffic_rec.get("xcd_chunk") in (None, kept["xcd_chunk"]):
             traffic = traffic_rec["l2_fabric_bytes_per_launch"]
 
     out = {
@@ -523,8 +525,8 @@ def main():
                 "bytes per launch between the L2 and the memory fabric (rocprofv3: FETCH_SIZE x 2 + WRITE_SIZE; "
                 "every L2 read request is 128 B on gfx950, TCC_EA0_RDREQ x 128 B equals FETCH_SIZE x 2); it "
                 "includes hits in the 256 MB last-level cache (MALL), which no L2-side counter separates from "
-                f"DRAM (TCC_EA0_RDREQ_DRAM equals TCC_EA0_RDREQ); measured on build {traffic_rec.get('build')}, "
-                f"{traffic_rec.get('source')}"),
+                f"DRAM (TCC_EA0_RDREQ_DRAM equals TCC_EA0_RDREQ); measured on build {traffic_rec.get('build')} with "
+                f"{traffic_rec.get('xcd_chunk')} tiles per XCD chunk, {traffic_rec.get('source')}"),
             "kernel": "rk4_stage (RHS + RK4 stage epilogue), 4 launches per step",
             "alg_bytes_per_launch": alg_bytes_per_launch,
             "avg_launch_ms": per_launch_s * 1e3,
