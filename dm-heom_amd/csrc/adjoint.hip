@@ -321,6 +321,10 @@ void check_adjoint_supported(heom_handle* h)
 {
     if (h->fp32())
         raise(HEOM_E_UNSUPPORTED, "the adjoint needs FP64 storage");
+    // its bath gradients are sums over the instances of one set of bath parameters
+    if (!h->uniform)
+        raise(HEOM_E_UNSUPPORTED,
+              "the adjoint needs a batch whose instances share states, modes, coupling and bath prefactors");
     if (h->nel > 64)
         raise(HEOM_E_UNSUPPORTED, "the adjoint needs N*N <= 64");
     // the transpose of a filtered hierarchy masks destinations instead of sources;

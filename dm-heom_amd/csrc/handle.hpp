@@ -93,6 +93,20 @@ struct AdjointState {
     DevBuf<double2> src;                         // [rows][count][nel] adjoint sources conj(w_r) O^dag
 };
 
+// The bath data of one class of instances: instances with the same states, coupling and
+// bath prefactors share one set of element constants (neighbour terms and "same"
+// constants) in a batch.
+struct BathClass {
+    int N = 0, K = 0, B = 0, BM = 0, W = 0;
+    std::vector<int32_t> coupling;              // [N * BM], < 0 = none
+    std::vector<double> cbk, pf_same, pf_circ;  // [2 B K], [2 B], [2 B]
+    bool operator==(const BathClass& o) const
+    {
+        return N == o.N && K == o.K && B == o.B && BM == o.BM && coupling == o.coupling && cbk == o.cbk &&
+               pf_same == o.pf_same && pf_circ == o.pf_circ;
+    }
+};
+
 // The thermal state search (tss.hip): the constants of the per-ADO transform.
 struct TssState {
     DevBuf<double2> c, ct;  // C and C^T [count][nel]
@@ -129,14 +143,23 @@ struct heom_handle {
     int pipe_wgs = 0;        // its workgroups: one per CU, a multiple of the XCD count
     int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
     int wide_ados = 0;       // N > 8: ADOs per workgroup of the wide kernel (0 = the most that fit)
+    int tile_split = 1;      // tile kernel: workgroups per 64-ADO tile (1, 2 or 4)
 
     // tuple-keyed readback (heom_get_ados): each instance's tuples in lexicographic
     // order [M][W] with the uid of each, and the current slot of every uid (-1: removed)
     std::vector<uint8_t> lex_tuples;
     std::vector<int32_t> lex_uid, u2q_host;
 
-    std::vector<int32_t> coupling;  // [N * BM]
+    std::vector<int32_t> coupling;  // [N * BM]  (bath class 0)
     std::vector<double> cbk, pf_same, pf_circ;
+    // Batches of instances that differ in states, modes, coupling or bath prefactors:
+    // every per-instance and per-ADO array uses the largest N and W (smaller instances
+    // padded with states that have no bath and stay zero, and modes no ADO uses); each
+    // instance belongs to a bath class whose element constants it uses.
+    std::vector<heom_rt::BathClass> classes;
+    std::vector<int32_t> inst_class;   // [count]
+    DevBuf<int32_t> tile_class, inst_class_d;  // [ntiles], [count]; empty with one class
+    bool uniform = true;               // one bath class and every instance of N states
 
     DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
     DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded

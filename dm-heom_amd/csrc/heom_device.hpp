@@ -22,10 +22,12 @@
 //     all lack plus neighbours (the deepest tier has none); a stage kernel skips the
 //     plus gathers of a tile whose flag says it has none (tile_plus).
 //
-//   * A batch of hierarchies with the same states and baths is one concatenated
-//     hierarchy: instance k owns the tiles from its tile offset on (its ADOs padded to
-//     a multiple of 64), its tables point into its own range, and a per-tile instance
-//     index selects its Hamiltonian.
+//   * A batch of hierarchies is one concatenated hierarchy: instance k owns the tiles
+//     from its tile offset on (its ADOs padded to a multiple of 64), its tables point
+//     into its own range, a per-tile instance index selects its Hamiltonian and a
+//     per-tile bath class its element constants.  Instances with fewer states or modes
+//     than the largest are padded with states that have no bath (they stay exactly
+//     zero) and with modes that no ADO uses.
 //
 // RK4 schedule (classical RK4, 12 state passes per step instead of 16): with
 // Y2 = y + h/2 k1, Y3 = y + h/2 k2, Y4 = y + h k3 and P = Y2 + 2 Y3 - y,
@@ -103,12 +105,14 @@ struct StageArgsT {
     const c2<R>* __restrict__ pfss;  // [Mp] pf_same_sum
     const int32_t* __restrict__ tile_inst;  // [ntiles] instance of each tile, or null (one instance)
     const int32_t* __restrict__ tile_plus;  // [ntiles] 0 if no ADO of the tile has a plus edge, or null
+    const int32_t* __restrict__ tile_class; // [ntiles] bath class of each tile (element constants), or null
     int64_t mp;
     int32_t ntiles;       // Mp / 64
     int32_t zero_slot;    // index of the zero slot (M)
     int32_t table_units;  // (W + 2) / 2: pairs of table rows 0..W (pipelined kernel)
     int32_t xcd_chunk;    // tiles per XCD chunk (chunk_tile)
     int32_t reverse;      // walk the tiles from the last one (tile t -> ntiles - 1 - t)
+    int32_t split;        // tile kernel: workgroups per tile (row blocks; 1, 2 or 4)
     R c1;                 // stage coefficient (h/2, h/2, h, h/6)
 };
 using StageArgs = StageArgsT<double>;
@@ -118,8 +122,8 @@ using StageArgs = StageArgsT<double>;
 template <typename R>
 struct ElementArgsT {
     const c2<R>* hs;        // [instances][N*N] H / hbar
-    const c2<R>* gam;       // [N*N] -(Delta_b summed over the baths of i and of j), i != j
-    const TermT<R>* terms;  // [N*N][NT] neighbour terms per element (padded)
+    const c2<R>* gam;       // [classes][N*N] -(Delta_b summed over the baths of i and of j), i != j
+    const TermT<R>* terms;  // [classes][N*N][NT] neighbour terms per element (padded)
 };
 using ElementArgs = ElementArgsT<double>;
 
@@ -148,6 +152,13 @@ template <typename R>
 __device__ __forceinline__ bool tile_has_plus(const StageArgsT<R>& a, int tile)
 {
     return a.tile_plus == nullptr || a.tile_plus[tile] != 0;
+}
+
+// the element constants (terms [classes][nel][nt], gam [classes][nel]) of a tile's bath class
+template <typename R>
+__device__ __forceinline__ int tile_bath_class(const StageArgsT<R>& a, int tile)
+{
+    return a.tile_class ? a.tile_class[tile] : 0;
 }
 
 // H / hbar of the instance a tile belongs to
@@ -268,6 +279,8 @@ __device__ __forceinline__ void stage_epilogue(const StageArgsT<R>& a, int64_t i
 bool stage_supported(int N, int nterms);
 // the padded term count the kernel for `nterms` expects
 int stage_term_slots(int nterms);
+// true if the tile kernel for N states can split a tile over `split` workgroups
+bool tile_split_supported(int N, int split);
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                   const ElementArgsT<double>& el, hipStream_t s);
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,

@@ -290,7 +290,7 @@ __global__ void mmult_left_rows(const double2* __restrict__ in, double2* __restr
 // For the top ADO and the W tier-1 ADOs this kernel then re-evaluates the RHS in FP64
 // and carries their RK4 state (y_n, the stage values and P) in FP64 shadows, writing
 // the rounded values into the planes for the neighbours to read.  Populations and
-// traces come from the FP64 top.  Thread = (tier-0/1 ADO s, element e).
+// traces come from the FP64 top.
 struct TierArgs {
     const float2* yin;    // stage input planes (neighbours that are not tier-0/1)
     float2* yout;         // stage output planes (overwritten at the tier-0/1 ADOs)
@@ -307,58 +307,93 @@ struct TierArgs {
     const double2* gam;   // [nel]
     const Term* terms;    // [nel][nt]
     const double2* pfss;  // [S] pf_same_sum
+    const int32_t* inst_class;  // [instances] bath class, or null (one class)
     int64_t mp;
     int32_t S, N, W, nt, zero;
     double c1;
 };
 
-template <int STAGE>
-__global__ void tier_fixup(TierArgs a)
+// One workgroup per tier-0/1 ADO s: its FP64 stage input, H / hbar of its instance and
+// its W neighbour-table entries (folded mask) go to LDS with coalesced loads; then each
+// thread takes elements e and issues the loads of all of the element's neighbour terms
+// (the FP64 shadow for a tier-0/1 neighbour, the complex64 plane otherwise) before the
+// commutator, which reads LDS, so the gathers' latency overlaps it.
+template <int STAGE, int NT>
+__global__ __launch_bounds__(256) void tier_fixup(TierArgs a)
 {
-    const int nel = a.N * a.N;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.S * nel)
-        return;
-    const int s = t / nel, e = t % nel, i = e / a.N, j = e % a.N;
+    extern __shared__ double2 tier_lds[];  // [nel] sigma, [nel] H / hbar, then [W] edges
+    const int N = a.N, nel = N * N, W = a.W;
+    // the W + 1 tier-0/1 ADOs of an instance run on one XCD, whose L2 then serves the
+    // tier-2 neighbours they share
+    const int s = xcd_tile(blockIdx.x, a.S, W + 1);
+    double2* sig = tier_lds;
+    double2* H = tier_lds + nel;
+    int4* edge = reinterpret_cast<int4*>(tier_lds + 2 * nel);  // {plus slot, minus slot, n, tier pair}
+    int2* spn = reinterpret_cast<int2*>(edge + W);             // tier-0/1 index of plus / minus, or -1
     const int64_t q = a.q[s];
-    const double2* sig = a.in64 + (int64_t)s * nel;
-    const double2* H = a.hs + (int64_t)a.inst[s] * nel;
-    double2 cm = make_double2(0.0, 0.0);
-    for (int k = 0; k < a.N; ++k) {
-        cm = cfma(sig[i * a.N + k], H[k * a.N + j], cm);
-        const double2 hik = H[i * a.N + k];
-        cm = cfma(make_double2(-hik.x, -hik.y), sig[k * a.N + j], cm);
+    const double2* in = a.in64 + (int64_t)s * nel;
+    const double2* hs = a.hs + (int64_t)a.inst[s] * nel;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        sig[e] = in[e];
+        H[e] = hs[e];
     }
-    const double2 S = a.pfss[s], g = a.gam[e];
-    double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), sig[e], make_double2(-cm.y, cm.x));
-    for (int k = 0; k < a.nt; ++k) {
-        const Term tm = a.terms[e * a.nt + k];
-        if (tm.mode >= a.W)
-            continue;  // padding
-        const int2 v = a.nb[(int64_t)tm.mode * a.mp + q];
-        const int32_t p = (int32_t)((uint32_t)v.x >> 6), m = v.y, n = v.x & 63;
-        const int2 sn = a.sp_nb[(int64_t)s * a.W + tm.mode];
-        double2 xp = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
-        if (p != a.zero)
-            xp = sn.x >= 0 ? a.in64[(int64_t)sn.x * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + p]);
-        if (m != a.zero)
-            xm = sn.y >= 0 ? a.in64[(int64_t)sn.y * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + m]);
-        const double cp = tm.plus_c * sqrt((double)(n + 1));
-        const double sm = sqrt((double)n);
-        f.x = fma(-cp, xp.y, f.x);
-        f.y = fma(cp, xp.x, f.y);
-        f = cfma(tm.minus_c, make_double2(sm * xm.x, sm * xm.y), f);
+    for (int m = threadIdx.x; m < W; m += blockDim.x) {
+        const int2 v = a.nb[(int64_t)m * a.mp + q];
+        edge[m] = make_int4((int32_t)((uint32_t)v.x >> 6), v.y, v.x & 63, 0);
+        spn[m] = a.sp_nb[(int64_t)s * W + m];
     }
-    const int64_t row = (int64_t)s * nel + e;
-    const int64_t pl = (int64_t)e * a.mp + q;
-    double2 r0, r1 = make_double2(0.0, 0.0);
-    const double2 ep = STAGE == 4 ? a.p64[row] : (STAGE == 2 || STAGE == 3) ? a.y64[row] : make_double2(0.0, 0.0);
-    stage_values<STAGE, double>(a.c1, sig[e], f, ep, r0, r1);
-    a.out64[row] = r0;
-    a.yout[pl] = make_float2((float)r0.x, (float)r0.y);
-    if (STAGE == 2) {
-        a.p64[row] = r1;
-        a.pout[pl] = make_float2((float)r1.x, (float)r1.y);
+    __syncthreads();
+    const double2 S = a.pfss[s];
+    const int cls = a.inst_class ? a.inst_class[a.inst[s]] : 0;
+    const Term* terms = a.terms + (int64_t)cls * nel * NT;
+    const double2* gam = a.gam + (int64_t)cls * nel;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        const int i = e / N, j = e - i * N;
+        double2 xp[NT], xm[NT];
+        int nv[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int mode = terms[e * NT + k].mode;
+            xp[k] = xm[k] = make_double2(0.0, 0.0);
+            nv[k] = 0;
+            if (mode < W) {
+                const int4 ed = edge[mode];
+                const int2 sn = spn[mode];
+                nv[k] = ed.z;
+                if (ed.x != a.zero)
+                    xp[k] = sn.x >= 0 ? a.in64[(int64_t)sn.x * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + ed.x]);
+                if (ed.y != a.zero)
+                    xm[k] = sn.y >= 0 ? a.in64[(int64_t)sn.y * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + ed.y]);
+            }
+        }
+        double2 cm = make_double2(0.0, 0.0);
+        for (int k = 0; k < N; ++k) {
+            cm = cfma(sig[i * N + k], H[k * N + j], cm);
+            const double2 hik = H[i * N + k];
+            cm = cfma(make_double2(-hik.x, -hik.y), sig[k * N + j], cm);
+        }
+        const double2 g = gam[e];
+        double2 f = cfma(make_double2(S.x + g.x, S.y + g.y), sig[e], make_double2(-cm.y, cm.x));
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const Term tm = terms[e * NT + k];
+            const double cp = tm.plus_c * sqrt((double)(nv[k] + 1));
+            const double sm = sqrt((double)nv[k]);
+            f.x = fma(-cp, xp[k].y, f.x);
+            f.y = fma(cp, xp[k].x, f.y);
+            f = cfma(tm.minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
+        }
+        const int64_t row = (int64_t)s * nel + e;
+        const int64_t pl = (int64_t)e * a.mp + q;
+        double2 r0, r1 = make_double2(0.0, 0.0);
+        const double2 ep = STAGE == 4 ? a.p64[row] : (STAGE == 2 || STAGE == 3) ? a.y64[row] : make_double2(0.0, 0.0);
+        stage_values<STAGE, double>(a.c1, sig[e], f, ep, r0, r1);
+        a.out64[row] = r0;
+        a.yout[pl] = make_float2((float)r0.x, (float)r0.y);
+        if (STAGE == 2) {
+            a.p64[row] = r1;
+            a.pout[pl] = make_float2((float)r1.x, (float)r1.y);
+        }
     }
 }
 
@@ -402,12 +437,14 @@ StageArgsT<R> stage_args(heom_handle* h, const void* yin, const void* y0, void* 
     a.pfss = pfss;
     a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
     a.tile_plus = h->tile_plus.p;
+    a.tile_class = h->uniform ? nullptr : h->tile_class.p;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
     a.zero_slot = h->zero_slot;
     a.table_units = (h->W + 2) / 2;
     a.xcd_chunk = h->xcd_chunk;
     a.reverse = reverse ? 1 : 0;
+    a.split = h->pipelined ? 1 : h->tile_split;
     a.c1 = (R)c1;
     return a;
 }
@@ -425,12 +462,14 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     a.pfss = pfss;
     a.tile_inst = h->count > 1 ? h->tile_inst.p : nullptr;
     a.tile_plus = h->tile_plus.p;
+    a.tile_class = h->uniform ? nullptr : h->tile_class.p;
     a.mp = h->Mp;
     a.ntiles = h->ntiles;
     a.zero_slot = h->zero_slot;
     a.table_units = (h->W + 2) / 2;
     a.xcd_chunk = h->xcd_chunk;
     a.reverse = reverse ? 1 : 0;
+    a.split = h->pipelined ? 1 : h->tile_split;
     a.c1 = (R)c1;
     if (h->N > kMaxStates)
         heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->wide_ados, h->stream);
@@ -465,6 +504,18 @@ void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void
     h->timing.rhs_launches += 1;
 }
 
+template <int NT>
+void tier_fixup_stage(int stage, dim3 grid, dim3 block, size_t lds, hipStream_t st, const TierArgs& a)
+{
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((tier_fixup<0, NT>), grid, block, lds, st, a); break;
+    case 1: hipLaunchKernelGGL((tier_fixup<1, NT>), grid, block, lds, st, a); break;
+    case 2: hipLaunchKernelGGL((tier_fixup<2, NT>), grid, block, lds, st, a); break;
+    case 3: hipLaunchKernelGGL((tier_fixup<3, NT>), grid, block, lds, st, a); break;
+    default: hipLaunchKernelGGL((tier_fixup<4, NT>), grid, block, lds, st, a); break;
+    }
+}
+
 // the FP64 pass over the tier-0/1 ADOs after a complex64 stage kernel
 void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsigned char* yout, double c1)
 {
@@ -486,6 +537,7 @@ void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsi
     a.gam = h->gam.p;
     a.terms = h->terms.p;
     a.pfss = h->sp_pfss.p;
+    a.inst_class = h->uniform ? nullptr : h->inst_class_d.p;
     a.mp = h->Mp;
     a.S = h->S;
     a.N = h->N;
@@ -493,13 +545,13 @@ void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsi
     a.nt = h->term_slots;
     a.zero = h->zero_slot;
     a.c1 = c1;
-    const dim3 grid(blocks_for((int64_t)h->S * h->nel, 64)), block(64);
-    switch (stage) {
-    case 0: hipLaunchKernelGGL(tier_fixup<0>, grid, block, 0, h->stream, a); break;
-    case 1: hipLaunchKernelGGL(tier_fixup<1>, grid, block, 0, h->stream, a); break;
-    case 2: hipLaunchKernelGGL(tier_fixup<2>, grid, block, 0, h->stream, a); break;
-    case 3: hipLaunchKernelGGL(tier_fixup<3>, grid, block, 0, h->stream, a); break;
-    default: hipLaunchKernelGGL(tier_fixup<4>, grid, block, 0, h->stream, a); break;
+    // one workgroup per tier-0/1 ADO, a wavefront for N <= 8
+    const dim3 grid(h->S), block(std::min(256, (h->nel + 63) / 64 * 64));
+    const size_t lds = 2 * (size_t)h->nel * sizeof(double2) + (size_t)h->W * (sizeof(int4) + sizeof(int2));
+    switch (h->term_slots) {
+    case 2: tier_fixup_stage<2>(stage, grid, block, lds, h->stream, a); break;
+    case 4: tier_fixup_stage<4>(stage, grid, block, lds, h->stream, a); break;
+    default: tier_fixup_stage<8>(stage, grid, block, lds, h->stream, a); break;
     }
     HIPC(hipGetLastError());
 }
@@ -646,14 +698,18 @@ void set_hamiltonians(heom_handle* h, const double* H)
     adjoint_invalidate(h);
 }
 
-int count_bath(const heom_handle* h, int s, int b)
+// coupling-list entries of state s equal to bath b (states past the class's own have none)
+int count_bath(const BathClass& c, int s, int b)
 {
-    int c = 0;
-    for (int t = 0; t < h->BM; ++t)
-        if (h->coupling[s * h->BM + t] == b)
-            ++c;
-    return c;
+    int n = 0;
+    if (s < c.N)
+        for (int t = 0; t < c.BM; ++t)
+            if (c.coupling[(size_t)s * c.BM + t] == b)
+                ++n;
+    return n;
 }
+
+int count_bath(const heom_handle* h, int s, int b) { return count_bath(h->classes[0], s, b); }
 
 // Neighbour terms per matrix element and the "same"-term constants.  For element
 // (i, j) and mode m of bath b = m / K, heom_ode.cl adds, for every coupling-list entry
@@ -666,49 +722,53 @@ int count_bath(const heom_handle* h, int s, int b)
 // minus_c = (i c_m/hbar (c_i - c_j) + [k=0] Phi_b (c_i + c_j)) sqrt(hbar/|c_m|).
 void build_coefficients(heom_handle* h)
 {
-    const int N = h->N, W = h->W, K = h->K, BM = h->BM;
-    std::vector<std::vector<Term>> per(N * N);
-    std::vector<double2> gam(N * N, make_double2(0.0, 0.0));
+    const int N = h->N, W = h->W, C = (int)h->classes.size();
+    std::vector<std::vector<Term>> per((size_t)C * N * N);
+    std::vector<double2> gam((size_t)C * N * N, make_double2(0.0, 0.0));
     int most = 0;
-    for (int i = 0; i < N; ++i) {
-        for (int j = 0; j < N; ++j) {
-            const int e = i * N + j;
-            for (int m = 0; m < W; ++m) {
-                const int b = m / K, k = m % K;
-                const int ci = count_bath(h, i, b), cj = count_bath(h, j, b);
-                const double cr = h->cbk[2 * m], cim = h->cbk[2 * m + 1];
-                const double a = std::hypot(cr, cim);
-                if (a == 0.0)
-                    continue;  // a mode with c_m = 0 contributes nothing (c sqrt(1/|c|) -> 0)
-                const double cp = std::sqrt(a / kHbar), cs = std::sqrt(kHbar / a);
-                Term t{};
-                t.mode = m;
-                t.plus_c = (ci - cj) * cp;
-                double br = -cim / kHbar * (ci - cj), bi = cr / kHbar * (ci - cj);
-                if (k == 0) {
-                    br += h->pf_circ[2 * b] * (ci + cj);
-                    bi += h->pf_circ[2 * b + 1] * (ci + cj);
-                }
-                t.minus_c = make_double2(br * cs, bi * cs);
-                if (t.plus_c != 0.0 || t.minus_c.x != 0.0 || t.minus_c.y != 0.0)
-                    per[e].push_back(t);
-            }
-            most = std::max(most, (int)per[e].size());
-            if (i != j) {
-                double gr = 0.0, gi = 0.0;
-                for (int t = 0; t < BM; ++t) {
-                    const int bi = h->coupling[i * BM + t];
-                    if (bi >= 0) {
-                        gr -= h->pf_same[2 * bi];
-                        gi -= h->pf_same[2 * bi + 1];
+    for (int cl = 0; cl < C; ++cl) {
+        const BathClass& bc = h->classes[cl];
+        for (int i = 0; i < N; ++i) {
+            for (int j = 0; j < N; ++j) {
+                const int e = i * N + j;
+                std::vector<Term>& pe = per[(size_t)cl * N * N + e];
+                for (int m = 0; m < bc.W; ++m) {
+                    const int b = m / bc.K, k = m % bc.K;
+                    const int ci = count_bath(bc, i, b), cj = count_bath(bc, j, b);
+                    const double cr = bc.cbk[2 * m], cim = bc.cbk[2 * m + 1];
+                    const double a = std::hypot(cr, cim);
+                    if (a == 0.0)
+                        continue;  // a mode with c_m = 0 contributes nothing (c sqrt(1/|c|) -> 0)
+                    const double cp = std::sqrt(a / kHbar), cs = std::sqrt(kHbar / a);
+                    Term t{};
+                    t.mode = m;
+                    t.plus_c = (ci - cj) * cp;
+                    double br = -cim / kHbar * (ci - cj), bi = cr / kHbar * (ci - cj);
+                    if (k == 0) {
+                        br += bc.pf_circ[2 * b] * (ci + cj);
+                        bi += bc.pf_circ[2 * b + 1] * (ci + cj);
                     }
-                    const int bj = h->coupling[j * BM + t];
-                    if (bj >= 0) {
-                        gr -= h->pf_same[2 * bj];
-                        gi -= h->pf_same[2 * bj + 1];
-                    }
+                    t.minus_c = make_double2(br * cs, bi * cs);
+                    if (t.plus_c != 0.0 || t.minus_c.x != 0.0 || t.minus_c.y != 0.0)
+                        pe.push_back(t);
                 }
-                gam[e] = make_double2(gr, gi);
+                most = std::max(most, (int)pe.size());
+                if (i != j) {
+                    double gr = 0.0, gi = 0.0;
+                    for (int t = 0; t < bc.BM; ++t) {
+                        const int bi = i < bc.N ? bc.coupling[(size_t)i * bc.BM + t] : -1;
+                        if (bi >= 0) {
+                            gr -= bc.pf_same[2 * bi];
+                            gi -= bc.pf_same[2 * bi + 1];
+                        }
+                        const int bj = j < bc.N ? bc.coupling[(size_t)j * bc.BM + t] : -1;
+                        if (bj >= 0) {
+                            gr -= bc.pf_same[2 * bj];
+                            gi -= bc.pf_same[2 * bj + 1];
+                        }
+                    }
+                    gam[(size_t)cl * N * N + e] = make_double2(gr, gi);
+                }
             }
         }
     }
@@ -717,8 +777,8 @@ void build_coefficients(heom_handle* h)
     h->term_slots = stage_term_slots(most);
     // pad every element's list; padding terms read table column W, which always
     // holds the zero slot
-    std::vector<Term> flat((size_t)N * N * h->term_slots);
-    for (int e = 0; e < N * N; ++e)
+    std::vector<Term> flat((size_t)C * N * N * h->term_slots);
+    for (int e = 0; e < C * N * N; ++e)
         for (int k = 0; k < h->term_slots; ++k) {
             Term t{};
             t.mode = W;
@@ -774,21 +834,14 @@ void validate(const heom_desc* d)
 }
 
 // the instances of a batch share states, baths, coupling and bath prefactors
+// The instances of a batch may differ in states, modes, coupling and bath prefactors
+// (they are padded to the largest N and W, and grouped into bath classes); they share
+// the storage precision.
 void validate_batch(const heom_desc* d, int count)
 {
-    for (int k = 1; k < count; ++k) {
-        const heom_desc& a = d[0];
-        const heom_desc& b = d[k];
-        if (a.states != b.states || a.matsubaras != b.matsubaras || a.baths != b.baths ||
-            a.baths_max_per_state != b.baths_max_per_state || a.width != b.width || a.precision != b.precision)
-            raise(HEOM_E_INVALID, "batch instances differ in states, baths, modes or precision");
-        const size_t nc = (size_t)a.states * a.baths_max_per_state;
-        if (!std::equal(a.coupling, a.coupling + nc, b.coupling) ||
-            !std::equal(a.pf_same, a.pf_same + 2 * a.baths, b.pf_same) ||
-            !std::equal(a.pf_circ, a.pf_circ + 2 * a.baths, b.pf_circ) ||
-            !std::equal(a.cbk, a.cbk + 2 * a.width, b.cbk))
-            raise(HEOM_E_INVALID, "batch instances differ in bath coupling or bath prefactors");
-    }
+    for (int k = 1; k < count; ++k)
+        if (d[k].precision != d[0].precision)
+            raise(HEOM_E_INVALID, "batch instances differ in precision");
 }
 
 int guard(const std::function<void()>& fn)
@@ -816,19 +869,31 @@ void need(heom_handle* h)
 }
 
 // The persistent pipelined stage kernel runs one workgroup per CU; it needs N >= 6 and
-// the neighbour-table slice of a tile in LDS.  HEOM_STAGE_KERNEL=tile in the
-// environment selects the one-workgroup-per-tile kernel instead.  The tile order
-// (chunk_tile in heom_device.hpp) defaults to one contiguous range per XCD for the
-// tile kernel and to XCD chunks of one tile per workgroup for the pipelined kernel;
-// HEOM_XCD_CHUNK=<tiles> overrides it.
+// the neighbour-table slice of a tile in LDS.  A hierarchy with fewer tiles than half the
+// CUs would leave most of the GPU idle with one workgroup per tile: it takes the tile
+// kernel with each tile split over 4 (or 2) workgroups where N allows.  In the
+// environment, HEOM_STAGE_KERNEL=tile selects the one-workgroup-per-tile kernel,
+// =split the tile kernel with the largest split, =pipelined the pipelined kernel where
+// it applies.  The tile order (chunk_tile in heom_device.hpp) defaults to one
+// contiguous range per XCD for the tile kernel and to XCD chunks of one tile per
+// workgroup for the pipelined kernel; HEOM_XCD_CHUNK=<tiles> overrides it.
 void choose_stage_kernel(heom_handle* h)
 {
     const char* env = std::getenv("HEOM_STAGE_KERNEL");
-    const bool want_tile = env && std::string(env) == "tile";
-    h->pipelined = !want_tile && pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
+    const std::string want = env ? env : "";
+    if (!want.empty() && want != "tile" && want != "split" && want != "pipelined")
+        raise(HEOM_E_INVALID, "HEOM_STAGE_KERNEL must be tile, split or pipelined");
+    int cus = 0;
+    HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    const bool small = 2 * h->ntiles <= cus;
+    h->tile_split = 1;
+    if (h->N <= kMaxStates && (want == "split" || (want.empty() && small)))
+        for (int sp : {4, 2})
+            if (h->tile_split == 1 && tile_split_supported(h->N, sp))
+                h->tile_split = sp;
+    h->pipelined = want != "tile" && want != "split" && h->tile_split == 1 &&
+                   pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
     if (h->pipelined) {
-        int cus = 0;
-        HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
         const int wgs = std::max(1, std::min(cus, h->ntiles));
         h->pipe_wgs = (wgs + kNumXcd - 1) / kNumXcd * kNumXcd;
         h->xcd_chunk = h->pipe_wgs / kNumXcd;
@@ -855,11 +920,12 @@ void setup_tiers(heom_handle* h, const heom_desc* d, const std::vector<std::vect
     h->sp_uid.clear();
     for (int k = 0; k < h->count; ++k) {
         const int32_t* T = d[k].ado_tuples;
+        const int Wk = d[k].width;
         local[k].assign(d[k].matrices, -1);
         for (int64_t u = 0; u < d[k].matrices; ++u) {
             int64_t sum = 0;
-            for (int m = 0; m < W; ++m)
-                sum += T[u * W + m];
+            for (int m = 0; m < Wk; ++m)
+                sum += T[u * Wk + m];
             if (sum > 1)
                 continue;
             local[k][u] = (int32_t)sq.size();
@@ -876,8 +942,9 @@ void setup_tiers(heom_handle* h, const heom_desc* d, const std::vector<std::vect
     for (int s = 0; s < h->S; ++s) {
         const int k = sinst[s];
         const int64_t u = h->sp_uid[s] - h->inst_u[k];
-        for (int m = 0; m < W; ++m) {
-            const int32_t p = d[k].ado_plus[u * W + m], mn = d[k].ado_minus[u * W + m];
+        const int Wk = d[k].width;
+        for (int m = 0; m < Wk; ++m) {
+            const int32_t p = d[k].ado_plus[u * Wk + m], mn = d[k].ado_minus[u * Wk + m];
             snb[(size_t)s * W + m] = make_int2(p >= 0 ? local[k][p] : -1, mn >= 0 ? local[k][mn] : -1);
         }
     }
@@ -901,19 +968,44 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPC(hipEventCreate(&h->ev_start));
     HIPC(hipEventCreate(&h->ev_stop));
-    const int N = d[0].states, W = d[0].width;
+    // the largest states and modes; bath classes of instances with equal bath data
+    int N = 0, W = 0;
+    h->classes.clear();
+    h->inst_class.assign(count, 0);
+    for (int k = 0; k < count; ++k) {
+        N = std::max(N, d[k].states);
+        W = std::max(W, d[k].width);
+        BathClass bc;
+        bc.N = d[k].states;
+        bc.K = d[k].matsubaras;
+        bc.B = d[k].baths;
+        bc.BM = d[k].baths_max_per_state;
+        bc.W = d[k].width;
+        bc.coupling.assign(d[k].coupling, d[k].coupling + (size_t)bc.N * bc.BM);
+        bc.cbk.assign(d[k].cbk, d[k].cbk + 2 * bc.W);
+        bc.pf_same.assign(d[k].pf_same, d[k].pf_same + 2 * bc.B);
+        bc.pf_circ.assign(d[k].pf_circ, d[k].pf_circ + 2 * bc.B);
+        const auto it = std::find(h->classes.begin(), h->classes.end(), bc);
+        h->inst_class[k] = (int32_t)(it - h->classes.begin());
+        if (it == h->classes.end())
+            h->classes.push_back(std::move(bc));
+    }
+    h->uniform = h->classes.size() == 1;
+    if (N > kMaxStates && !wide_supported(N, d[0].precision == HEOM_PRECISION_FP32 ? 8 : 16))
+        raise(HEOM_E_UNSUPPORTED, "states > 32 not supported by this build");
+    const BathClass& c0 = h->classes[0];
     h->N = N;
-    h->K = d[0].matsubaras;
-    h->B = d[0].baths;
-    h->BM = d[0].baths_max_per_state;
+    h->K = c0.K;
+    h->B = c0.B;
+    h->BM = c0.BM;
     h->W = W;
     h->nel = N * N;
     h->prec = d[0].precision;
     h->count = count;
-    h->coupling.assign(d[0].coupling, d[0].coupling + N * h->BM);
-    h->cbk.assign(d[0].cbk, d[0].cbk + 2 * h->B * h->K);
-    h->pf_same.assign(d[0].pf_same, d[0].pf_same + 2 * h->B);
-    h->pf_circ.assign(d[0].pf_circ, d[0].pf_circ + 2 * h->B);
+    h->coupling = c0.coupling;
+    h->cbk = c0.cbk;
+    h->pf_same = c0.pf_same;
+    h->pf_circ = c0.pf_circ;
 
     // instance k: uids [inst_u, inst_u + M_k), internal slots from inst_q (a multiple of
     // 64); the zero slot follows the last instance
@@ -953,33 +1045,48 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     for (int k = 0; k < count; ++k) {
         const int64_t M = d[k].matrices, qb = h->inst_q[k];
         const int32_t* T = d[k].ado_tuples;
-        for (int64_t t = 0; t < M * W; ++t)
+        const int Wk = d[k].width;  // modes Wk..W-1 (padding) are unused by this instance
+        for (int64_t t = 0; t < M * Wk; ++t)
             if (T[t] < 0 || T[t] > kMaxTupleValue)
                 raise(HEOM_E_UNSUPPORTED, "tuple entries must be in [0, 62]");
-        for (int m = 0; m < W; ++m)
+        for (int m = 0; m < Wk; ++m)
             if (T[m] != 0)
                 raise(HEOM_E_INVALID, "uid 0 must be the top ADO (all-zero tuple)");
         // lexicographic ADO order (mode 0 most significant)
         std::vector<int32_t> order(M);
         std::iota(order.begin(), order.end(), 0);
         std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            return std::lexicographical_compare(T + (int64_t)a * W, T + (int64_t)a * W + W, T + (int64_t)b * W,
-                                                T + (int64_t)b * W + W);
+            return std::lexicographical_compare(T + (int64_t)a * Wk, T + (int64_t)a * Wk + Wk, T + (int64_t)b * Wk,
+                                                T + (int64_t)b * Wk + Wk);
         });
         for (int64_t q = 0; q < M; ++q) {
             // the sorted tuples of heom_get_ados: lexicographic position -> uid
             const int64_t lx = h->inst_u[k] + q;
-            for (int m = 0; m < W; ++m)
-                h->lex_tuples[(size_t)lx * W + m] = (uint8_t)T[(int64_t)order[q] * W + m];
+            for (int m = 0; m < Wk; ++m)
+                h->lex_tuples[(size_t)lx * W + m] = (uint8_t)T[(int64_t)order[q] * Wk + m];
             h->lex_uid[(size_t)lx] = (int32_t)(h->inst_u[k] + order[q]);
         }
-        // slots: the lexicographic order grouped into tiles with and without plus edges
-        std::vector<uint8_t> has_plus(M, 0);
-        for (int64_t u = 0; u < M; ++u)
-            for (int m = 0; m < W; ++m)
-                if (d[k].ado_plus[u * W + m] >= 0)
+        // slots: the ADOs of tier <= 2 first (with FP32 storage the FP64 pass over the
+        // tier-0/1 ADOs reads their tier-2 neighbours, which then share cache lines), then
+        // the others, each part in lexicographic order, grouped into tiles with and
+        // without plus edges
+        std::vector<uint8_t> has_plus(M, 0), low(M, 0);
+        for (int64_t u = 0; u < M; ++u) {
+            int64_t tier = 0;
+            for (int m = 0; m < Wk; ++m) {
+                tier += T[u * Wk + m];
+                if (d[k].ado_plus[u * Wk + m] >= 0)
                     has_plus[u] = 1;
-        const std::vector<int32_t> slots = group_tiles(order, has_plus);
+            }
+            low[u] = tier <= 2;
+        }
+        std::vector<int32_t> seq;
+        seq.reserve(M);
+        for (int pass = 1; pass >= 0; --pass)
+            for (int64_t q = 0; q < M; ++q)
+                if (low[order[q]] == pass)
+                    seq.push_back(order[q]);
+        const std::vector<int32_t> slots = group_tiles(seq, has_plus);
         std::vector<int32_t>& u2q = u2q_all[k];
         u2q.assign(M, 0);
         for (int64_t q = 0; q < M; ++q) {
@@ -992,14 +1099,14 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
             tinst[t] = k;
         for (int64_t q = 0; q < M; ++q) {
             const int64_t u = slots[q];
-            for (int m = 0; m < W; ++m) {
-                const int32_t p = d[k].ado_plus[u * W + m];
-                const int32_t mn = d[k].ado_minus[u * W + m];
+            for (int m = 0; m < Wk; ++m) {
+                const int32_t p = d[k].ado_plus[u * Wk + m];
+                const int32_t mn = d[k].ado_minus[u * Wk + m];
                 if (p >= M || mn >= M)
                     raise(HEOM_E_INVALID, "plus/minus entry out of range");
                 const int32_t pq = p >= 0 ? (int32_t)(qb + u2q[p]) : zero;
-                const int32_t mq = (mn >= 0 && T[u * W + m] > 0) ? (int32_t)(qb + u2q[mn]) : zero;
-                nb[(size_t)m * Mp + qb + q] = make_int2((int32_t)(((uint32_t)pq << 6) | (uint32_t)T[u * W + m]), mq);
+                const int32_t mq = (mn >= 0 && T[u * Wk + m] > 0) ? (int32_t)(qb + u2q[mn]) : zero;
+                nb[(size_t)m * Mp + qb + q] = make_int2((int32_t)(((uint32_t)pq << 6) | (uint32_t)T[u * Wk + m]), mq);
             }
             pfss[qb + q] = make_double2(d[k].pf_same_sum[2 * u], d[k].pf_same_sum[2 * u + 1]);
         }
@@ -1011,6 +1118,13 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     upload(h->q2u, q2u, h->stream);
     upload(h->tile_inst, tinst, h->stream);
     upload(h->q_top, qtop, h->stream);
+    if (!h->uniform) {
+        std::vector<int32_t> tcls(h->ntiles, h->inst_class[count - 1]);
+        for (int t = 0; t < h->ntiles; ++t)
+            tcls[t] = h->inst_class[tinst[t]];
+        upload(h->tile_class, tcls, h->stream);
+        upload(h->inst_class_d, h->inst_class, h->stream);
+    }
     if (h->fp32()) {
         std::vector<float2> p32(Mp);
         for (int64_t q = 0; q < Mp; ++q)
@@ -1028,9 +1142,14 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
         HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
     }
     build_coefficients(h);
-    std::vector<double> H((size_t)count * h->nel * 2);
+    // H of each instance, padded to N x N
+    std::vector<double> H((size_t)count * h->nel * 2, 0.0);
     for (int k = 0; k < count; ++k)
-        std::copy(d[k].hamiltonian, d[k].hamiltonian + 2 * h->nel, H.begin() + (size_t)k * h->nel * 2);
+        for (int i = 0; i < d[k].states; ++i)
+            for (int j = 0; j < d[k].states; ++j)
+                for (int c = 0; c < 2; ++c)
+                    H[((size_t)k * h->nel + (size_t)i * N + j) * 2 + c] =
+                        d[k].hamiltonian[((size_t)i * d[k].states + j) * 2 + c];
     set_hamiltonians(h, H.data());
     if (h->fp32())
         setup_tiers(h, d, u2q_all);
@@ -1140,7 +1259,7 @@ void mmult_left_planes(heom_handle* h)
 
 // One configuration of the stage kernels (the fields of heom_tune_result).
 struct StageConfig {
-    int kind, chunk, wgs, ados;
+    int kind, chunk, wgs, ados, split = 1;
 };
 
 // every configuration the autotuner times for this handle: for N <= 8 the tile kernel
@@ -1165,6 +1284,9 @@ std::vector<StageConfig> stage_candidates(heom_handle* h)
     } else {
         for (int c : chunks(std::max(1, nt / kNumXcd)))
             cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
+        for (int sp : {2, 4})  // tiles split over workgroups, at the default chunk
+            if (tile_split_supported(h->N, sp))
+                cands.push_back({HEOM_STAGE_TILE, std::max(1, nt / kNumXcd), 0, kTile, sp});
         if (pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes())) {
             int cus = 0;
             HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -1182,6 +1304,7 @@ void apply_stage_config(heom_handle* h, const StageConfig& c)
     h->pipe_wgs = c.kind == HEOM_STAGE_PIPELINED ? c.wgs : h->pipe_wgs;
     h->xcd_chunk = c.chunk;
     h->wide_ados = c.kind == HEOM_STAGE_WIDE ? c.ados : 0;
+    h->tile_split = c.kind == HEOM_STAGE_TILE ? c.split : 1;
     drop_graph(h);
 }
 
@@ -2042,6 +2165,7 @@ int heom_autotune(heom_handle* h, int32_t steps, heom_tune_result* results, int3
             results[k].xcd_chunk = cands[k].chunk;
             results[k].workgroups = cands[k].wgs;
             results[k].ados_per_workgroup = cands[k].ados;
+            results[k].tile_split = cands[k].split;
             results[k].chosen = k == best ? 1 : 0;
             results[k].ms_per_step = ms[k];
         }
@@ -2055,10 +2179,12 @@ int heom_set_stage_config(heom_handle* h, const heom_tune_result* config)
         reject_tss(h, "heom_set_stage_config");
         if (!config)
             raise(HEOM_E_INVALID, "config is NULL");
-        const StageConfig c{config->kernel, config->xcd_chunk, config->workgroups, config->ados_per_workgroup};
+        const StageConfig c{config->kernel, config->xcd_chunk, config->workgroups, config->ados_per_workgroup,
+                            config->tile_split < 1 ? 1 : config->tile_split};
         bool ok = false;
         for (const StageConfig& v : stage_candidates(h))
-            if (v.kind == c.kind && v.ados == c.ados && (c.kind != HEOM_STAGE_PIPELINED || v.wgs == c.wgs))
+            if (v.kind == c.kind && v.ados == c.ados && v.split == c.split &&
+                (c.kind != HEOM_STAGE_PIPELINED || v.wgs == c.wgs))
                 ok = true;
         if (!ok)
             raise(HEOM_E_INVALID, "stage configuration (kernel " + std::to_string(c.kind) + ", " +

@@ -38,19 +38,24 @@ namespace heom_hip {
 
 namespace {
 
-// One tile; PLUS = false leaves out the plus gathers (no ADO of the tile has a plus
+// One tile, or with SPLIT > 1 row block rb of SPLIT of it (small hierarchies: SPLIT
+// workgroups share a tile, each loading all of it into LDS for the commutator's columns
+// and computing 1 / SPLIT of its wavefronts, so that more CUs share the tile's memory
+// instructions); PLUS = false leaves out the plus gathers (no ADO of the tile has a plus
 // neighbour, so they would all read the zero slot).
-template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ, bool PLUS>
+template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ, bool PLUS, int SPLIT>
 __device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* __restrict__ hs_all,
                                            const c2<R>* __restrict__ gam, const TermT<R>* __restrict__ terms,
-                                           int tile, c2<R>* sig)
+                                           int tile, int rb, c2<R>* sig)
 {
     using C = c2<R>;
     constexpr int CW = kStageColumns;
     constexpr int NJW = (N + CW - 1) / CW;  // wavefronts per row
+    constexpr int WPG = stage_waves(N) / SPLIT;  // wavefronts of this workgroup
 
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wl = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = rb * WPG + wl;  // this wavefront's place among the tile's
     const int i = w / NJW;
     const int jb = (w % NJW) * CW;
     const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
@@ -66,6 +71,18 @@ __device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* 
             sig[(i * N + j) * kTile + lane] = s[c];
         } else {
             s[c] = mk2<R>(R(0), R(0));
+        }
+    }
+    // the elements of the other row blocks, for the commutator's columns
+#pragma unroll
+    for (int r = 1; r < SPLIT; ++r) {
+        const int w2 = ((rb + r) % SPLIT) * WPG + wl;
+        const int i2 = w2 / NJW, jb2 = (w2 % NJW) * CW;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            const int j = jb2 + c;
+            if (j < N)
+                sig[(i2 * N + j) * kTile + lane] = a.yin[(int64_t)(i2 * N + j) * mp + q];
         }
     }
     // with few terms per element, the table entries and the epilogue operands of all
@@ -167,31 +184,52 @@ __device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* 
     }
 }
 
-template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ>
-__global__ __launch_bounds__(64 * stage_waves(N)) void rk4_stage(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
-                                                                  const c2<R>* __restrict__ gam,
-                                                                  const TermT<R>* __restrict__ terms)
+template <typename R, int N, int NT, int STAGE, bool REALH, bool ADJ, int SPLIT>
+__global__ __launch_bounds__(64 * stage_waves(N) / SPLIT) void rk4_stage(StageArgsT<R> a,
+                                                                          const c2<R>* __restrict__ hs_all,
+                                                                          const c2<R>* __restrict__ gam,
+                                                                          const TermT<R>* __restrict__ terms)
 {
     __shared__ c2<R> sig[N * N * kTile];
-    const int t0 = xcd_tile(blockIdx.x, a.ntiles, a.xcd_chunk);
+    const int t0 = xcd_tile(blockIdx.x / SPLIT, a.ntiles, a.xcd_chunk);
+    const int rb = blockIdx.x % SPLIT;
     const int tile = a.reverse ? a.ntiles - 1 - t0 : t0;
+    const int cls = tile_bath_class(a, tile);
+    const c2<R>* gam_c = gam + cls * (N * N);
+    const TermT<R>* terms_c = terms + cls * (N * N * NT);
     if (tile_has_plus(a, tile))
-        stage_tile<R, N, NT, STAGE, REALH, ADJ, true>(a, hs_all, gam, terms, tile, sig);
+        stage_tile<R, N, NT, STAGE, REALH, ADJ, true, SPLIT>(a, hs_all, gam_c, terms_c, tile, rb, sig);
     else
-        stage_tile<R, N, NT, STAGE, REALH, ADJ, false>(a, hs_all, gam, terms, tile, sig);
+        stage_tile<R, N, NT, STAGE, REALH, ADJ, false, SPLIT>(a, hs_all, gam_c, terms_c, tile, rb, sig);
 }
 
+template <typename R, int N, int NT, bool REALH, bool ADJ, int SPLIT>
+void launch_s(int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
+{
+    const dim3 grid(a.ntiles * SPLIT), block(64 * stage_waves(N) / SPLIT);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((rk4_stage<R, N, NT, 0, REALH, ADJ, SPLIT>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 1: hipLaunchKernelGGL((rk4_stage<R, N, NT, 1, REALH, ADJ, SPLIT>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 2: hipLaunchKernelGGL((rk4_stage<R, N, NT, 2, REALH, ADJ, SPLIT>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    case 3: hipLaunchKernelGGL((rk4_stage<R, N, NT, 3, REALH, ADJ, SPLIT>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    default: hipLaunchKernelGGL((rk4_stage<R, N, NT, 4, REALH, ADJ, SPLIT>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    }
+}
+
+// row splits are compiled for N >= 5 (ten or more wavefronts per tile), where they divide
+// the tile's wavefronts (tile_split_supported)
 template <typename R, int N, int NT, bool REALH, bool ADJ>
 void launch_t(int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, hipStream_t st)
 {
-    const dim3 grid(a.ntiles), block(64 * stage_waves(N));
-    switch (stage) {
-    case 0: hipLaunchKernelGGL((rk4_stage<R, N, NT, 0, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 1: hipLaunchKernelGGL((rk4_stage<R, N, NT, 1, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 2: hipLaunchKernelGGL((rk4_stage<R, N, NT, 2, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    case 3: hipLaunchKernelGGL((rk4_stage<R, N, NT, 3, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
-    default: hipLaunchKernelGGL((rk4_stage<R, N, NT, 4, REALH, ADJ>), grid, block, 0, st, a, el.hs, el.gam, el.terms); break;
+    if constexpr (N >= 5 && stage_waves(N) % 4 == 0) {
+        if (a.split == 4)
+            return launch_s<R, N, NT, REALH, ADJ, 4>(stage, a, el, st);
     }
+    if constexpr (N >= 5 && stage_waves(N) % 2 == 0) {
+        if (a.split == 2)
+            return launch_s<R, N, NT, REALH, ADJ, 2>(stage, a, el, st);
+    }
+    launch_s<R, N, NT, REALH, ADJ, 1>(stage, a, el, st);
 }
 
 template <typename R, int N, bool REALH, bool ADJ>
@@ -241,6 +279,11 @@ void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const 
 int stage_term_slots(int nterms) { return nterms <= 2 ? 2 : nterms <= 4 ? 4 : 8; }
 
 bool stage_supported(int N, int nterms) { return N >= 1 && N <= kMaxStates && nterms >= 0 && nterms <= 8; }
+
+bool tile_split_supported(int N, int split)
+{
+    return split == 1 || (N >= 5 && N <= kMaxStates && (split == 2 || split == 4) && stage_waves(N) % split == 0);
+}
 
 void launch_stage(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                   const ElementArgsT<double>& el, hipStream_t s)

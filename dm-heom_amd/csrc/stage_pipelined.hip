@@ -115,6 +115,9 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
         return p + z;
     };
     const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
+    const int cls = tile_bath_class(a, tile);
+    gam += cls * (N * N);
+    terms += cls * (N * N * NT);
     const C S_ = a.pfss[q];
     auto slot = [](int k, int j) { return (k * S::RS + j) * kTile; };
 

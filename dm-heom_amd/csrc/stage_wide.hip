@@ -62,19 +62,32 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
 {
     using C = c2<R>;
     __shared__ C lds[LDSB / sizeof(C)];
+    __shared__ R sqrt_n[64];  // sqrt(n), n = 0..63
     const int nel = N * N, NP = wide_row(N), mat = N * NP;
     C* hl = lds;         // [N][NP] H / hbar
     C* sig = lds + mat;  // [T][N][NP] the tile, then commutator + same term
+    // index arithmetic without runtime divisions: T is a power of two, and
+    // e / N = (e * ninv) >> 20 for every e < 8256 and N <= 32
+    const int tsh = __builtin_ctz((unsigned)T);
+    const uint32_t ninv = (1u << 20) / (uint32_t)N + 1u;
+    auto div_n = [&](int e) { return (int)(((uint32_t)e * ninv) >> 20); };
 
     const int64_t mp = a.mp;
     // XCD-aware order: runs of 16 consecutive workgroups (16 T ADOs) on one XCD
     const int64_t q0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x, 16) * T;
     const C* hs = hs_all + (a.tile_inst ? (int64_t)a.tile_inst[q0 / kTile] * nel : 0);
-    for (int idx = threadIdx.x; idx < nel; idx += blockDim.x)
-        hl[(idx / N) * NP + idx % N] = hs[idx];
+    const int cls = tile_bath_class(a, (int)(q0 / kTile));
+    gam += cls * nel;
+    terms += cls * nel * NT;
+    if (threadIdx.x < 64)
+        sqrt_n[threadIdx.x] = sqrt((R)threadIdx.x);
+    for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) {
+        const int r = div_n(idx);
+        hl[r * NP + idx - r * N] = hs[idx];
+    }
     for (int idx = threadIdx.x; idx < T * nel; idx += blockDim.x) {
-        const int ad = idx % T, e = idx / T;
-        sig[ad * mat + (e / N) * NP + e % N] = a.yin[(int64_t)e * mp + q0 + ad];
+        const int ad = idx & (T - 1), e = idx >> tsh, r = div_n(e);
+        sig[ad * mat + r * NP + e - r * N] = a.yin[(int64_t)e * mp + q0 + ad];
     }
     __syncthreads();
 
@@ -157,9 +170,9 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
                 const int idx = base + u * stride;
                 live[u] = idx < total;
                 const int id = live[u] ? idx : base;  // a dead second pair repeats the first's loads
-                const int ad = id % T, e = id / T;
+                const int ad = id & (T - 1), e = id >> tsh, r = div_n(e);
                 e2[u] = e;
-                lo[u] = ad * mat + (e / N) * NP + e % N;
+                lo[u] = ad * mat + r * NP + e - r * N;
                 o[u] = (int64_t)e * mp + q0 + ad;
 #pragma unroll
                 for (int k = 0; k < NT; ++k)
@@ -187,11 +200,11 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
                     const TermT<R> t = terms[e2[u] * NT + k];
                     const int v = nbv[u][k].x & 63;
                     if constexpr (PLUS) {
-                        const R cp = t.plus_c * sqrt((R)(v + 1));
+                        const R cp = t.plus_c * sqrt_n[v + 1];
                         f.x = fma(-cp, xp[u][k].y, f.x);
                         f.y = fma(cp, xp[u][k].x, f.y);
                     }
-                    const R sm = sqrt((R)v);
+                    const R sm = sqrt_n[v];
                     f = cfma(t.minus_c, mk2<R>(sm * xm[u][k].x, sm * xm[u][k].y), f);
                 }
                 C r0, r1 = mk2<R>(R(0), R(0));

@@ -74,7 +74,7 @@ class Gradients(ctypes.Structure):
 
 class TuneResult(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32), ("workgroups", ctypes.c_int32),
-                ("ados_per_workgroup", ctypes.c_int32), ("chosen", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ados_per_workgroup", ctypes.c_int32), ("chosen", ctypes.c_int32), ("tile_split", ctypes.c_int32),
                 ("ms_per_step", ctypes.c_double)]
 
 
@@ -225,10 +225,12 @@ class Handle:
         _check(load().heom_create_batch(arr, len(descs), ctypes.byref(h)))
         self._h = h
         self.count = len(descs)
-        self.states = descs[0].states
+        # a batch may mix state counts and modes: arrays use the largest of each
+        self.instance_states = [int(d.states) for d in descs]
+        self.states = max(self.instance_states)
         self.instance_matrices = [int(d.matrices) for d in descs]
         self.matrices = sum(self.instance_matrices)
-        self.width = descs[0].width
+        self.width = max(int(d.width) for d in descs)
         self.baths = descs[0].baths
         self.matsubaras = descs[0].matsubaras
         self.precision = descs[0].precision
@@ -251,7 +253,8 @@ class Handle:
 
     @classmethod
     def from_instances(cls, insts, precision=HEOM_PRECISION_FP64) -> "Handle":
-        """A batch handle over several instances with the same states and baths."""
+        """A batch handle over several instances (any states, baths and bath parameters;
+        arrays of the batch use the largest state count)."""
         made = [_make_desc(precision=precision, **_instance_arrays(i)) for i in insts]
         return cls._create([d for d, _ in made], [k for _, k in made])
 
@@ -358,7 +361,19 @@ class Handle:
         _check(load().heom_set_state(self._h, s.ctypes.data))
 
     def _per_instance(self, m) -> np.ndarray:
-        """[count][N][N] complex from one N x N matrix (used for every instance) or a stack."""
+        """[count][N][N] complex from one N x N matrix (used for every instance), a stack,
+        or a list with one matrix per instance of that instance's own size (padded with
+        zeros to the batch's N)."""
+        if isinstance(m, (list, tuple)) and len(m) == self.count and any(
+                np.shape(x) != (self.states, self.states) for x in m):
+            out = np.zeros((self.count, self.states, self.states), dtype=np.complex128)
+            for k, x in enumerate(m):
+                x = np.asarray(x)
+                n = self.instance_states[k]
+                if x.shape != (n, n):
+                    raise ValueError(f"instance {k} expects a {n} x {n} matrix")
+                out[k, :n, :n] = x
+            return _c128_flat(out)
         m = np.asarray(m)
         if m.ndim == 2:
             m = np.broadcast_to(m, (self.count,) + m.shape)
@@ -408,6 +423,8 @@ class Handle:
     def get_ados(self, tuples, instance: int = 0) -> np.ndarray:
         """The ADOs with the given tuples ([n, W] ints) of one instance: complex [n, N, N]."""
         t = np.ascontiguousarray(tuples, dtype=np.int32)
+        if t.ndim == 2 and t.shape[1] < self.width:  # an instance with fewer modes than the batch
+            t = np.ascontiguousarray(np.pad(t, ((0, 0), (0, self.width - t.shape[1]))))
         if t.ndim != 2 or t.shape[1] != self.width:
             raise ValueError("tuples must be [n, W]")
         out = np.zeros(t.shape[0] * self.states * self.states * 2)
@@ -525,7 +542,8 @@ class Handle:
         n = ctypes.c_int32()
         _check(load().heom_autotune(self._h, int(steps), res, cap, ctypes.byref(n)))
         return [{"kernel": STAGE_KINDS[r.kernel], "xcd_chunk": r.xcd_chunk, "workgroups": r.workgroups,
-                 "ados_per_workgroup": r.ados_per_workgroup, "chosen": bool(r.chosen), "ms_per_step": r.ms_per_step}
+                 "ados_per_workgroup": r.ados_per_workgroup, "tile_split": r.tile_split, "chosen": bool(r.chosen),
+                 "ms_per_step": r.ms_per_step}
                 for r in res[:min(n.value, cap)]]
 
     def set_stage_config(self, config: dict) -> None:
@@ -533,7 +551,8 @@ class Handle:
         workgroups, ados_per_workgroup) without timing anything."""
         kinds = {v: k for k, v in STAGE_KINDS.items()}
         r = TuneResult(kernel=kinds[config["kernel"]], xcd_chunk=int(config["xcd_chunk"]),
-                       workgroups=int(config["workgroups"]), ados_per_workgroup=int(config["ados_per_workgroup"]))
+                       workgroups=int(config["workgroups"]), ados_per_workgroup=int(config["ados_per_workgroup"]),
+                       tile_split=int(config.get("tile_split", 1)))
         _check(load().heom_set_stage_config(self._h, ctypes.byref(r)))
 
     # adjoint

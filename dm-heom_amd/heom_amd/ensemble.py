@@ -4,9 +4,11 @@ one batch handle (heom_create_batch), population dynamics and linear absorption.
 The reference runs its apps once per realisation (dm-heom/src/app_population_dynamics
 .cpp, app_linear_absorption.cpp).  Here each realisation's config file goes through
 the same host path (config -> hierarchy graph -> instance), and all realisations are
-stepped together: one batch of population hierarchies (the sites) and one batch of
-absorption hierarchies (the sites plus the ground state, one instance per dipole
-tensor component).  Populations and absorption traces are reduced on the device.
+stepped together in one batch handle: the population hierarchies (the 7 sites, padded
+to 8 states by the batch with a state that has no bath and stays zero) and the
+absorption hierarchies (the sites plus the ground state, one instance per dipole tensor
+component), one kernel launch per RK4 stage for the whole ensemble.  Populations and
+absorption traces are reduced on the device.
 
 Linear absorption at a 2 fs step needs the rotating frame: the optical coherences
 rotate at |E| (about 12,000 cm^-1, omega dt = 4.5), beyond classical RK4's stability
@@ -75,15 +77,19 @@ class Ensemble:
                 out.extend([d] * repeat)
             return out
 
-        self.pop = Handle._create(descs(self.pop_hosts, 1), keepalive=self.pop_hosts)
-        self.la = Handle._create(descs(self.la_hosts, self.ncomp), keepalive=self.la_hosts)
-        for h in (self.pop, self.la):
-            h.use_graphs(use_graphs)
+        # instances 0 .. members-1: populations; then ncomp absorption instances per member
+        self.handle = Handle._create(descs(self.pop_hosts, 1) + descs(self.la_hosts, self.ncomp),
+                                     keepalive=self.pop_hosts + self.la_hosts)
+        self.handle.use_graphs(use_graphs)
         self.pop_ados = sum(h.matrices for h in self.pop_hosts)
         self.la_ados = sum(h.matrices for h in self.la_hosts) * self.ncomp
 
-        # rotating frame: each realisation's mean site energy leaves the excited block
+        # rotating frame: each realisation's mean site energy leaves the excited block of
+        # its absorption instances; the population instances keep their Hamiltonian
         Hs, self.e0 = [], []
+        for hst in self.pop_hosts:
+            H = hst.arrays()["hamiltonian"].reshape(hst.states, hst.states, 2)
+            Hs.append(H[..., 0] + 1j * H[..., 1])
         for hst in self.la_hosts:
             H = hst.arrays()["hamiltonian"].reshape(hst.states, hst.states, 2)
             Hc = H[..., 0] + 1j * H[..., 1]
@@ -92,52 +98,50 @@ class Ensemble:
             Hc[np.arange(1, hst.states), np.arange(1, hst.states)] -= e0
             self.e0.append(e0)
             Hs.extend([Hc] * self.ncomp)
-        self.la.set_hamiltonian(np.array(Hs))
+        self.handle.set_hamiltonian(Hs)
         self.dminus, self.dplus = [], []
         for hst in self.la_hosts:
             for t in range(self.ncomp):
                 dm, dp = hst.dipoles(t)
                 self.dminus.append(dm)
                 self.dplus.append(dp)
-        self.dminus, self.dplus = np.array(self.dminus), np.array(self.dplus)
-        # stage-kernel configurations timed on the batches themselves (heom_autotune)
+        eye = [np.eye(h.states) for h in self.pop_hosts]
+        zero = [np.zeros((h.states, h.states)) for h in self.pop_hosts]
+        self.start_ops = eye + self.dplus    # D+ on the absorption instances, identity elsewhere
+        self.trace_ops = zero + self.dminus  # Tr(D- sigma_0) on the absorption instances
+        # the stage-kernel configuration timed on the batch itself (heom_autotune)
         self.tuning = {}
         if autotune:
             self.reset()
-            for name, h in (("populations", self.pop), ("absorption", self.la)):
-                self.tuning[name] = next(r for r in h.autotune(steps=3) if r["chosen"])
+            self.tuning["ensemble"] = next(r for r in self.handle.autotune(steps=3) if r["chosen"])
 
     def reset(self) -> None:
         """Initial states: |site_1><site_1| (populations) and D+ |g><g| (absorption)."""
-        self.pop.clear_state()
-        self.pop.set_top(np.array([h.top() for h in self.pop_hosts]))
-        self.la.clear_state()
-        self.la.set_top(np.array([h.top() for h in self.la_hosts for _ in range(self.ncomp)]))
-        self.la.mmult_left(self.dplus)
+        self.handle.clear_state()
+        self.handle.set_top([h.top() for h in self.pop_hosts] +
+                            [h.top() for h in self.la_hosts for _ in range(self.ncomp)])
+        self.handle.mmult_left(self.start_ops)
 
     def run(self, steps: int | None = None) -> dict:
-        """Propagate both batches; populations (rows, members, sites) and the averaged
-        absorption correlation C(t) (rows, members), with device times."""
+        """Propagate the ensemble; populations (rows, members, sites) and the averaged
+        absorption correlation C(t) (rows, members), with the device time."""
         steps = self.steps if steps is None else steps
         self.reset()
-        pops, _ = self.pop.propagate_observe(self.step_size, steps, self.observe_steps)
-        t_pop = self.pop.timing().last_call_seconds
-        _, tr = self.la.propagate_observe(self.step_size, steps, self.observe_steps, self.dminus)
-        t_la = self.la.timing().last_call_seconds
+        pops, tr = self.handle.propagate_observe(self.step_size, steps, self.observe_steps, self.trace_ops)
+        t = self.handle.timing().last_call_seconds
         rows = pops.shape[0]
+        m, sites = self.members, self.pop_hosts[0].states
         times = np.arange(rows) * self.observe_steps * self.step_size
         phase = np.exp(-1j * np.outer(times, np.array(self.e0)) / HBAR)
-        corr = tr.reshape(rows, self.members, self.ncomp).mean(axis=2) * phase
+        corr = tr[:, m:].reshape(rows, m, self.ncomp).mean(axis=2) * phase
         ados = self.pop_ados + self.la_ados
         return {
-            "times": times, "populations": pops, "absorption": corr, "steps": steps,
-            "device_seconds": t_pop + t_la, "pop_seconds": t_pop, "la_seconds": t_la,
-            "ado_steps": ados * steps, "ados": ados,
+            "times": times, "populations": pops[:, :m, :sites], "absorption": corr, "steps": steps,
+            "device_seconds": t, "ado_steps": ados * steps, "ados": ados,
         }
 
     def close(self) -> None:
-        for h in (self.pop, self.la):
-            h.close()
+        self.handle.close()
         for hst in self.pop_hosts + self.la_hosts:
             hst.close()
         self._tmp.cleanup()

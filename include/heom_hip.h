@@ -91,13 +91,18 @@ typedef struct heom_timing_info {
  * of desc to the device.  The initial state is all zeros and the mask all TRUE. */
 int heom_create(const heom_desc* desc, heom_handle** out);
 /* A batch of `count` hierarchies in one handle (no counterpart in the reference, whose
- * apps run one instance at a time; for ensembles such as static-disorder
- * realisations).  The instances must share states, matsubaras, baths, coupling, the
- * bath prefactors (pf_same, pf_circ, cbk) and precision; their Hamiltonians and
- * depths may differ.  They are stepped together: one kernel launch per RK4 stage for
- * the whole batch.  Per-instance arguments below ("[count]" arrays: top, hamiltonian,
- * mmult_left matrices, recorded tops) are indexed by instance; per-ADO arrays (state,
- * mask, norms, rhs) are the instances' arrays one after the other.  heom_create is
+ * apps run one instance at a time; for ensembles such as static-disorder realisations
+ * and parameter scans).  The instances share the precision; their Hamiltonians, depths,
+ * bath prefactors (pf_same, pf_circ, cbk), coupling, baths, matsubaras and states may
+ * differ.  They are stepped together: one kernel launch per RK4 stage for the whole
+ * batch.  Per-instance arguments below ("[count]" arrays: top, hamiltonian, mmult_left
+ * matrices, observables, recorded tops and populations) are indexed by instance; per-ADO
+ * arrays (state, mask, norms, rhs) are the instances' arrays one after the other.
+ * Every such array uses the batch's N = the largest `states` (and tuples the largest
+ * `width`): an instance with fewer states has them first, in its own order, and rows
+ * and columns past them that stay exactly zero (states without a bath and without
+ * Hamiltonian couplings, which leave the instance's own block unchanged); an instance
+ * with fewer modes has its tuples first and zeros after them.  heom_create is
  * heom_create_batch with count = 1. */
 int heom_create_batch(const heom_desc* descs, int32_t count, heom_handle** out);
 int heom_instances(heom_handle* handle, int32_t* count);
@@ -237,8 +242,11 @@ int heom_rk4_step_bytes(heom_handle* handle, int64_t* bytes);
 /* which stage kernel the handle runs (no counterpart in the reference; for tests and
  * benchmarks): HEOM_STAGE_TILE = one workgroup per 64-ADO tile, HEOM_STAGE_PIPELINED =
  * persistent workgroups streaming tiles through LDS (N >= 6 and few bath modes),
- * HEOM_STAGE_WIDE = one wavefront per ADO for 8 < N <= 32.  The environment variable
- * HEOM_STAGE_KERNEL=tile at heom_create forces the tile kernel for N <= 8. */
+ * HEOM_STAGE_WIDE = one wavefront per ADO for 8 < N <= 32.  A hierarchy of fewer
+ * 64-ADO tiles than half the CUs runs the tile kernel with each tile split over 2 or 4
+ * workgroups.  The environment variable HEOM_STAGE_KERNEL at heom_create overrides the
+ * choice for N <= 8: tile (one workgroup per tile), split (the tile kernel, largest
+ * split) or pipelined (where it applies). */
 enum heom_stage_kind { HEOM_STAGE_TILE = 0, HEOM_STAGE_PIPELINED = 1, HEOM_STAGE_WIDE = 2 };
 int heom_stage_kernel(heom_handle* handle, int* kind);
 
@@ -253,8 +261,9 @@ int heom_stage_kernel(heom_handle* handle, int* kind);
 int heom_compact(heom_handle* handle, int64_t* active);
 
 /* The autotuner (no counterpart in the reference): times every valid configuration of
- * the stage kernels for this handle -- tile or pipelined kernel and tiles per XCD chunk
- * of the tile order (N <= 8), ADOs per workgroup (N > 8, HEOM_STAGE_WIDE) -- over `steps`
+ * the stage kernels for this handle -- tile or pipelined kernel, tiles per XCD chunk of
+ * the tile order and workgroups per tile of the tile kernel (N <= 8), ADOs per workgroup
+ * (N > 8, HEOM_STAGE_WIDE) -- over `steps`
  * RK4 steps each on the handle's own state (restored afterwards) and keeps the fastest.
  * Writes min(count, capacity) results; count receives the number of candidates. */
 typedef struct heom_tune_result {
@@ -263,7 +272,7 @@ typedef struct heom_tune_result {
     int32_t workgroups;          /* pipelined kernel: persistent workgroups, else 0 */
     int32_t ados_per_workgroup;  /* 64, or the wide kernel's ADOs per workgroup */
     int32_t chosen;              /* 1 for the configuration kept */
-    int32_t reserved;
+    int32_t tile_split;          /* tile kernel: workgroups per 64-ADO tile (1, 2 or 4), else 1 */
     double ms_per_step;          /* device time of one RK4 step */
 } heom_tune_result;
 int heom_autotune(heom_handle* handle, int32_t steps, heom_tune_result* results, int32_t capacity,
@@ -279,7 +288,8 @@ int heom_set_stage_config(heom_handle* handle, const heom_tune_result* config);
 /* ---- adjoint: gradients of observables ------------------------------------------ */
 /* No counterpart in the reference, which has no differentiation; for fitting
  * Hamiltonians and bath parameters to measured dynamics and spectra.  FP64 handles with
- * an unfiltered hierarchy (no masked ADOs), N*N <= 64.
+ * an unfiltered hierarchy (no masked ADOs), N*N <= 64, and in a batch instances that share
+ * states, modes, coupling and bath prefactors (the bath gradients are their sums).
  *
  * The objective is J = sum_r sum_k Re(w_rk X_rk) with X_rk = Tr(O_k sigma_0,k(t_r)), the
  * observable O_k on the top ADO of instance k at t_r = r * observe_steps * h,

@@ -13,7 +13,11 @@
  *   partitions / graph   pinned: closed-form counts, brute-force enumeration, a step-by-step
  *                        restatement of the reference generator, plus/minus invariants
  *   precompute           pinned: 50-digit mpmath evaluation of the formulas; Pade table
- *                        checked against the Bose function it approximates
+ *                        checked against the Bose function it approximates; c_k, nu_k,
+ *                        Delta_b and Phi_b against the Drude-Lorentz correlation function
+ *                        computed by Fourier quadrature of J(w) (the exponentials converge
+ *                        to Re C(t) with every Pade term, Delta_b is the zero-frequency
+ *                        weight they miss, Phi_b gives Im C(t)); planted factors fail
  *   rhs                  pinned: dense-Liouvillian brute force, no-bath von Neumann limit
  *                        (expm), trace conservation, Hermiticity
  *   rk4                  pinned: order-4 convergence against expm of the dense Liouvillian;

@@ -15,7 +15,12 @@ pieces the reference keeps in C++:
 * the instance assembly (``src/heom/instance.cpp:25-69, 483-495``);
 * the dipole matrices of the linear-absorption pseudo pathway
   (``include/heom/dipole_matrix.hpp:30-74``, ``src/heom/dipole_matrix.cpp:182-208``,
-  ``src/app_linear_absorption.cpp:88-163``);
+  ``src/app_linear_absorption.cpp:88-163``), pinned with the absorption driver in
+  tests/test_oracle_physics.py by the bath-free limit (the dipole correlation from
+  scipy's expm and the config's directions), the exact second-cumulant line shape of a
+  two-level monomer, and the rotating frame differing from the plain one by RK4
+  truncation error only (falling as dt^4); planted sign errors in D+ or in the frame
+  phase fail;
 * the thermal state search set-up and loop
   (``include/heom/thermal_state_search_solver.hpp:124-325``,
   ``src/app_thermal_state_search.cpp:84-128``), with numpy's eigh in place of the

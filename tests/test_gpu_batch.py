@@ -107,14 +107,15 @@ def test_batch_rhs_state_mask_and_mmult(oracle_mod):
         offm += inst.matrices
 
 
-def test_batch_rejects_different_baths(oracle_mod):
+def test_batch_rejects_mixed_precision(oracle_mod):
+    """Instances may differ in baths and states (tests/test_gpu_batch_mixed.py), not in
+    storage precision."""
     a = instance(7, 3, 1)
     b = instance(7, 3, 2, lam=40.0)
-    with pytest.raises(heom_amd.HeomError, match="bath"):
-        heom_amd.Handle.from_instances([a, b])
-    c = instance(6, 3, 3)
-    with pytest.raises(heom_amd.HeomError, match="differ"):
-        heom_amd.Handle.from_instances([a, c])
+    descs = [heom_amd._make_desc(precision=p, **heom_amd._instance_arrays(i))
+             for i, p in ((a, heom_amd.HEOM_PRECISION_FP64), (b, heom_amd.HEOM_PRECISION_FP32))]
+    with pytest.raises(heom_amd.HeomError, match="precision"):
+        heom_amd.Handle._create([d for d, _ in descs], [k for _, k in descs])
 
 
 def test_graph_replays_match_eager(oracle_mod):

@@ -18,12 +18,9 @@ heom_amd = pytest.importorskip("heom_amd")
 FP32 = heom_amd.HEOM_PRECISION_FP32
 
 
-@pytest.fixture(params=["pipelined", "tile"], autouse=True)
+@pytest.fixture(params=["pipelined", "tile", "split"], autouse=True)
 def stage_kernel(request, monkeypatch):
-    if request.param == "tile":
-        monkeypatch.setenv("HEOM_STAGE_KERNEL", "tile")
-    else:
-        monkeypatch.delenv("HEOM_STAGE_KERNEL", raising=False)
+    monkeypatch.setenv("HEOM_STAGE_KERNEL", request.param)
     return request.param
 
 

@@ -16,14 +16,13 @@ pytestmark = pytest.mark.gpu
 heom_amd = pytest.importorskip("heom_amd")
 
 
-@pytest.fixture(params=["pipelined", "tile"], autouse=True)
+@pytest.fixture(params=["pipelined", "tile", "split"], autouse=True)
 def stage_kernel(request, monkeypatch):
-    """Every test runs with both stage kernels: the persistent pipelined one (chosen by
-    default where it applies: N >= 6, few modes) and the one-workgroup-per-tile one."""
-    if request.param == "tile":
-        monkeypatch.setenv("HEOM_STAGE_KERNEL", "tile")
-    else:
-        monkeypatch.delenv("HEOM_STAGE_KERNEL", raising=False)
+    """Every test runs with the three stage-kernel configurations for N <= 8: the
+    persistent pipelined kernel (where it applies: N >= 6, few modes), the
+    one-workgroup-per-tile kernel, and the tile kernel with each tile split over
+    workgroups (the default for small hierarchies; where N allows)."""
+    monkeypatch.setenv("HEOM_STAGE_KERNEL", request.param)
     return request.param
 
 
