@@ -144,6 +144,8 @@ struct heom_handle {
     int xcd_chunk = 1;       // tiles per XCD chunk of the stage kernels' tile order
     int wide_ados = 0;       // N > 8: ADOs per workgroup of the wide kernel (0 = the most that fit)
     int tile_split = 1;      // tile kernel: workgroups per 64-ADO tile (1, 2 or 4)
+    bool paired = false;     // complex64 planes in the paired layout (stage_pair.hip)
+    int pair_slots = 0;      // neighbour terms per plane of the paired layout (0: not possible)
 
     // tuple-keyed readback (heom_get_ados): each instance's tuples in lexicographic
     // order [M][W] with the uid of each, and the current slot of every uid (-1: removed)
@@ -173,6 +175,7 @@ struct heom_handle {
     // FP32 storage
     DevBuf<float2> pfss32, hs32, gam32;
     DevBuf<TermT<float>> terms32;
+    DevBuf<heom_hip::PairTerm> pterms;  // [classes][pair_planes(N)][pair_slots]
     int S = 0;                    // tier-0/1 ADOs
     std::vector<int32_t> sp_top;  // [count] tier-0/1 index of each top
     std::vector<int64_t> sp_uid;  // [S] global uid

@@ -127,6 +127,42 @@ struct ElementArgsT {
 };
 using ElementArgs = ElementArgsT<double>;
 
+// ---- paired complex64 layout ---------------------------------------------------------
+//
+// With FP32 storage and the pipelined stage kernel (stage_pair.hip), the elements (i, j)
+// and (j, i), i < j, share one plane of 16-byte slots (halves 0 and 1), and so do the
+// diagonal elements (2a, 2a) and (2a + 1, 2a + 1) (an odd N leaves the last diagonal
+// alone, half 1 stays zero).  Both elements of a pair couple to the same bath modes, so
+// one 16-byte gather of a neighbour serves both.  A plane holds Mp slots.
+__host__ __device__ constexpr int pair_count(int N) { return N * (N - 1) / 2; }
+__host__ __device__ constexpr int pair_planes(int N) { return pair_count(N) + (N + 1) / 2; }
+// plane (x) and half (y) of element (i, j)
+__host__ __device__ __forceinline__ int2 pair_slot(int N, int i, int j)
+{
+    if (i == j)
+        return make_int2(pair_count(N) + (i >> 1), i & 1);
+    const int a = i < j ? i : j, b = i < j ? j : i;
+    return make_int2(a * (2 * N - a - 1) / 2 + (b - a - 1), i < j ? 0 : 1);
+}
+// index, in complex64 units, of element e of ADO q: paired ((plane * mp + q) * 2 + half)
+// or one plane per element (e * mp + q)
+__host__ __device__ __forceinline__ int64_t c64_index(bool paired, int N, int e, int64_t q, int64_t mp)
+{
+    if (!paired)
+        return (int64_t)e * mp + q;
+    const int2 s = pair_slot(N, e / N, e % N);
+    return ((int64_t)s.x * mp + q) * 2 + s.y;
+}
+
+// The neighbour terms of one plane of the paired layout: the union of its two elements'
+// modes, each with both elements' constants (zero where an element lacks the mode).
+struct PairTerm {
+    float2 minus_c[2];
+    float plus_c[2];
+    int32_t mode;
+    int32_t info;
+};
+
 // ---- helpers -------------------------------------------------------------------------
 
 __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c)  // c + a*b
@@ -303,6 +339,14 @@ void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const Sta
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
                        const ElementArgsT<float>& el, int ados, hipStream_t s);
 
+// ---- paired pipelined launcher (stage_pair.hip), complex64 state --------------------
+
+// true if the paired kernel handles N states with W modes and Mp ADO slots
+bool pair_supported(int N, int W, int64_t mp);
+// el.terms is unused; pterms holds [classes][pair_planes(N)][nterm_slots] plane terms
+void launch_stage_pair(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
+                       const ElementArgsT<float>& el, const PairTerm* pterms, int workgroups, hipStream_t s);
+
 // ---- pipelined launcher (stage_pipelined.hip) -----------------------------------------
 
 // LDS row stride (element slots per matrix row) of the pipelined kernel's tile: the
@@ -324,7 +368,5 @@ bool pipe_supported(int N, int W, int64_t mp, int complex_bytes);
 // grid: `workgroups` persistent workgroups, a multiple of kNumXcd
 void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                             const ElementArgsT<double>& el, int workgroups, hipStream_t s);
-void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
-                            const ElementArgsT<float>& el, int workgroups, hipStream_t s);
 
 }  // namespace heom_hip

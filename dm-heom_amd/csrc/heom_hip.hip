@@ -21,6 +21,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "handle.hpp"
@@ -60,23 +61,33 @@ __device__ __forceinline__ float2 from_d2<float2>(double2 v)
     return make_float2((float)v.x, (float)v.y);
 }
 
-// dst[e * mp + q] = src[u(q)][e]   (reference AoS -> internal SoA; padding -> 0)
+// Element e of ADO q in the planes: c64_index (heom_device.hpp); complex128 planes are
+// never paired.
+template <typename C>
+__device__ __forceinline__ int64_t plane_index(bool paired, int N, int e, int64_t q, int64_t mp)
+{
+    return c64_index(sizeof(C) == sizeof(float2) && paired, N, e, q, mp);
+}
+
+// dst[e, q] = src[u(q)][e]   (reference AoS -> internal SoA; padding -> 0)
 template <typename C>
 __global__ void aos_to_soa(const double2* __restrict__ src, C* __restrict__ dst, const int32_t* __restrict__ q2u,
-                           int64_t mp, int nel)
+                           int64_t mp, int N, bool paired)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= mp)
         return;
     const int32_t u = q2u[q];
+    const int nel = N * N;
     for (int e = 0; e < nel; ++e)
-        dst[(int64_t)e * mp + q] = from_d2<C>(u >= 0 ? src[(int64_t)u * nel + e] : make_double2(0.0, 0.0));
+        dst[plane_index<C>(paired, N, e, q, mp)] =
+            from_d2<C>(u >= 0 ? src[(int64_t)u * nel + e] : make_double2(0.0, 0.0));
 }
 
-// dst[u(q)][e] = src[e * mp + q]   (internal SoA -> reference AoS)
+// dst[u(q)][e] = src[e, q]   (internal SoA -> reference AoS)
 template <typename C>
 __global__ void soa_to_aos(const C* __restrict__ src, double2* __restrict__ dst, const int32_t* __restrict__ q2u,
-                           int64_t mp, int nel)
+                           int64_t mp, int N, bool paired)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= mp)
@@ -84,31 +95,34 @@ __global__ void soa_to_aos(const C* __restrict__ src, double2* __restrict__ dst,
     const int32_t u = q2u[q];
     if (u < 0)
         return;
+    const int nel = N * N;
     for (int e = 0; e < nel; ++e)
-        dst[(int64_t)u * nel + e] = to_d2(src[(int64_t)e * mp + q]);
+        dst[(int64_t)u * nel + e] = to_d2(src[plane_index<C>(paired, N, e, q, mp)]);
 }
 
-// dst[k][e] = src[e * mp + q[k]]: one ADO per instance (the tops), thread = (k, e)
+// dst[k][e] = src[e, q[k]]: one ADO per instance (the tops), thread = (k, e)
 template <typename C>
 __global__ void gather_ados(const C* __restrict__ src, double2* __restrict__ dst, const int64_t* __restrict__ qs,
-                            int count, int64_t mp, int nel)
+                            int count, int64_t mp, int N, bool paired)
 {
+    const int nel = N * N;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * nel)
         return;
     const int k = t / nel, e = t % nel;
-    dst[t] = to_d2(src[(int64_t)e * mp + qs[k]]);
+    dst[t] = to_d2(src[plane_index<C>(paired, N, e, qs[k], mp)]);
 }
 
 template <typename C>
 __global__ void scatter_ados(const double2* __restrict__ src, C* __restrict__ dst, const int64_t* __restrict__ qs,
-                             int count, int64_t mp, int nel)
+                             int count, int64_t mp, int N, bool paired)
 {
+    const int nel = N * N;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count * nel)
         return;
     const int k = t / nel, e = t % nel;
-    dst[(int64_t)e * mp + qs[k]] = from_d2<C>(src[t]);
+    dst[plane_index<C>(paired, N, e, qs[k], mp)] = from_d2<C>(src[t]);
 }
 
 // rows of the FP64 shadow: dst[k][e] = shadow[idx[k]][e]
@@ -167,7 +181,7 @@ __global__ __launch_bounds__(64) void observe_tops(const double2* __restrict__ t
 // dm-heom/cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
 template <typename C>
 __global__ void hierarchy_norm_kernel(const C* __restrict__ y, const int32_t* __restrict__ q2u, int64_t mp,
-                                      int nel, double threshold, double* __restrict__ norms,
+                                      int N, bool paired, double threshold, double* __restrict__ norms,
                                       uint8_t* __restrict__ mask)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,8 +191,8 @@ __global__ void hierarchy_norm_kernel(const C* __restrict__ y, const int32_t* __
     if (u < 0)
         return;
     double acc = 0.0;
-    for (int e = 0; e < nel; ++e) {
-        const double2 v = to_d2(y[(int64_t)e * mp + q]);
+    for (int e = 0; e < N * N; ++e) {
+        const double2 v = to_d2(y[plane_index<C>(paired, N, e, q, mp)]);
         acc = fma(v.x, v.x, acc);
         acc = fma(v.y, v.y, acc);
     }
@@ -220,7 +234,8 @@ __global__ void remap_planes(const double2* __restrict__ src, double2* __restric
 // tile in LDS, arithmetic in FP64
 template <typename C, int N>
 __global__ __launch_bounds__(64 * N) void mmult_left_kernel(C* __restrict__ y, const double2* __restrict__ mx,
-                                                            const int32_t* __restrict__ tile_inst, int64_t mp)
+                                                            const int32_t* __restrict__ tile_inst, int64_t mp,
+                                                            bool paired)
 {
     __shared__ double2 t[N * N * kTile];
     const int lane = threadIdx.x & 63;
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(64 * N) void mmult_left_kernel(C* __restrict__ y, c
     const double2* m_k = mx + (tile_inst ? (int64_t)tile_inst[blockIdx.x] * N * N : 0);
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        t[(i * N + j) * kTile + lane] = to_d2(y[(int64_t)(i * N + j) * mp + q]);
+        t[(i * N + j) * kTile + lane] = to_d2(y[plane_index<C>(paired, N, i * N + j, q, mp)]);
     __syncthreads();
     double2 out[N];
 #pragma unroll
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(64 * N) void mmult_left_kernel(C* __restrict__ y, c
     }
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        y[(int64_t)(i * N + j) * mp + q] = from_d2<C>(out[j]);
+        y[plane_index<C>(paired, N, i * N + j, q, mp)] = from_d2<C>(out[j]);
 }
 
 // mmult_left for N > 8, out of place: out[e][q] = sum_k mx_ik y[k*N+j][q], thread = (e, q)
@@ -308,6 +323,7 @@ struct TierArgs {
     const Term* terms;    // [nel][nt]
     const double2* pfss;  // [S] pf_same_sum
     const int32_t* inst_class;  // [instances] bath class, or null (one class)
+    int32_t paired;       // the complex64 planes are in the paired layout
     int64_t mp;
     int32_t S, N, W, nt, zero;
     double c1;
@@ -361,9 +377,11 @@ __global__ __launch_bounds__(256) void tier_fixup(TierArgs a)
                 const int2 sn = spn[mode];
                 nv[k] = ed.z;
                 if (ed.x != a.zero)
-                    xp[k] = sn.x >= 0 ? a.in64[(int64_t)sn.x * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + ed.x]);
+                    xp[k] = sn.x >= 0 ? a.in64[(int64_t)sn.x * nel + e]
+                                      : to_d2(a.yin[c64_index(a.paired, N, e, ed.x, a.mp)]);
                 if (ed.y != a.zero)
-                    xm[k] = sn.y >= 0 ? a.in64[(int64_t)sn.y * nel + e] : to_d2(a.yin[(int64_t)e * a.mp + ed.y]);
+                    xm[k] = sn.y >= 0 ? a.in64[(int64_t)sn.y * nel + e]
+                                      : to_d2(a.yin[c64_index(a.paired, N, e, ed.y, a.mp)]);
             }
         }
         double2 cm = make_double2(0.0, 0.0);
@@ -384,7 +402,7 @@ __global__ __launch_bounds__(256) void tier_fixup(TierArgs a)
             f = cfma(tm.minus_c, make_double2(sm * xm[k].x, sm * xm[k].y), f);
         }
         const int64_t row = (int64_t)s * nel + e;
-        const int64_t pl = (int64_t)e * a.mp + q;
+        const int64_t pl = c64_index(a.paired, N, e, q, a.mp);
         double2 r0, r1 = make_double2(0.0, 0.0);
         const double2 ep = STAGE == 4 ? a.p64[row] : (STAGE == 2 || STAGE == 3) ? a.y64[row] : make_double2(0.0, 0.0);
         stage_values<STAGE, double>(a.c1, sig[e], f, ep, r0, r1);
@@ -471,12 +489,17 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
     a.reverse = reverse ? 1 : 0;
     a.split = h->pipelined ? 1 : h->tile_split;
     a.c1 = (R)c1;
-    if (h->N > kMaxStates)
+    if (h->N > kMaxStates) {
         heom_hip::launch_stage_wide(h->N, h->term_slots, h->real_h, stage, a, el, h->wide_ados, h->stream);
-    else if (h->pipelined)
-        heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
-    else
+    } else if (h->pipelined) {
+        if constexpr (std::is_same<R, float>::value)
+            heom_hip::launch_stage_pair(h->N, h->pair_slots, h->real_h, stage, a, el, h->pterms.p, h->pipe_wgs,
+                                        h->stream);
+        else
+            heom_hip::launch_stage_pipelined(h->N, h->term_slots, h->real_h, stage, a, el, h->pipe_wgs, h->stream);
+    } else {
         heom_hip::launch_stage(h->N, h->term_slots, h->real_h, stage, a, el, h->stream);
+    }
     HIPC(hipGetLastError());
 }
 
@@ -538,6 +561,7 @@ void tier_fixup_launch(heom_handle* h, int stage, const unsigned char* yin, unsi
     a.terms = h->terms.p;
     a.pfss = h->sp_pfss.p;
     a.inst_class = h->uniform ? nullptr : h->inst_class_d.p;
+    a.paired = h->paired ? 1 : 0;
     a.mp = h->Mp;
     a.S = h->S;
     a.N = h->N;
@@ -720,6 +744,62 @@ int count_bath(const heom_handle* h, int s, int b) { return count_bath(h->classe
 // With c_i, c_j the number of entries equal to b in rows i and j, one Term per mode
 // carries plus_c = (c_i - c_j) sqrt(|c_m|/hbar) and
 // minus_c = (i c_m/hbar (c_i - c_j) + [k=0] Phi_b (c_i + c_j)) sqrt(hbar/|c_m|).
+// The plane terms of the paired layout: per class and plane, the union of its two
+// elements' modes with both elements' constants (zero where an element lacks the mode),
+// padded with mode W.  Sets pair_slots to 0 when the layout does not apply (N outside
+// 4..8, or more than 8 terms on a plane).
+void build_pair_terms(heom_handle* h, const std::vector<std::vector<Term>>& per, int C)
+{
+    const int N = h->N, W = h->W, D = pair_planes(N);
+    h->pair_slots = 0;
+    if (N < 4 || N > kMaxStates)
+        return;
+    std::vector<std::vector<PairTerm>> planes((size_t)C * D);
+    int most = 0;
+    for (int cl = 0; cl < C; ++cl)
+        for (int i = 0; i < N; ++i)
+            for (int j = i; j < N; ++j) {
+                // the plane's elements: (i, j) with (j, i), or the diagonals (i, i), (i+1, i+1)
+                if (i == j && (i & 1))
+                    continue;
+                const bool diag = i == j;
+                const int e0 = i * N + j, e1 = diag ? (i + 1 < N ? (i + 1) * N + i + 1 : -1) : j * N + i;
+                const int d = pair_slot(N, i, j).x;
+                std::vector<PairTerm>& out = planes[(size_t)cl * D + d];
+                for (int half = 0; half < 2; ++half) {
+                    const int e = half ? e1 : e0;
+                    if (e < 0)
+                        continue;
+                    for (const Term& t : per[(size_t)cl * N * N + e]) {
+                        auto it = std::find_if(out.begin(), out.end(),
+                                               [&](const PairTerm& pt) { return pt.mode == t.mode; });
+                        if (it == out.end()) {
+                            PairTerm z{};
+                            z.mode = t.mode;
+                            out.push_back(z);
+                            it = out.end() - 1;
+                        }
+                        it->plus_c[half] = (float)t.plus_c;
+                        it->minus_c[half] = make_float2((float)t.minus_c.x, (float)t.minus_c.y);
+                    }
+                }
+                most = std::max(most, (int)out.size());
+            }
+    if (most > 8)
+        return;
+    h->pair_slots = stage_term_slots(most);
+    std::vector<PairTerm> flat((size_t)C * D * h->pair_slots);
+    for (size_t pl = 0; pl < planes.size(); ++pl)
+        for (int k = 0; k < h->pair_slots; ++k) {
+            PairTerm pad{};
+            pad.mode = W;
+            PairTerm& dst = flat[pl * h->pair_slots + k];
+            dst = k < (int)planes[pl].size() ? planes[pl][k] : pad;
+            dst.info = (int32_t)planes[pl].size();
+        }
+    upload(h->pterms, flat, h->stream);
+}
+
 void build_coefficients(heom_handle* h)
 {
     const int N = h->N, W = h->W, C = (int)h->classes.size();
@@ -804,6 +884,7 @@ void build_coefficients(heom_handle* h)
             g32[e] = make_float2((float)gam[e].x, (float)gam[e].y);
         upload(h->terms32, f32, h->stream);
         upload(h->gam32, g32, h->stream);
+        build_pair_terms(h, per, C);
     }
 }
 
@@ -877,22 +958,48 @@ void need(heom_handle* h)
 // it applies.  The tile order (chunk_tile in heom_device.hpp) defaults to one
 // contiguous range per XCD for the tile kernel and to XCD chunks of one tile per
 // workgroup for the pipelined kernel; HEOM_XCD_CHUNK=<tiles> overrides it.
-void choose_stage_kernel(heom_handle* h)
+std::string stage_kernel_request()
 {
     const char* env = std::getenv("HEOM_STAGE_KERNEL");
     const std::string want = env ? env : "";
     if (!want.empty() && want != "tile" && want != "split" && want != "pipelined")
         raise(HEOM_E_INVALID, "HEOM_STAGE_KERNEL must be tile, split or pipelined");
+    return want;
+}
+
+int device_cus(heom_handle* h)
+{
     int cus = 0;
     HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    return cus;
+}
+
+// FP32 storage with N <= 8 runs the pipelined kernel in the paired layout (stage_pair.hip)
+// where it applies, and the tile kernel on one plane per element otherwise; the choice
+// fixes the layout of the planes for the handle's lifetime.
+bool use_paired_layout(heom_handle* h)
+{
+    if (!h->fp32() || h->N > kMaxStates || h->pair_slots == 0 || !pair_supported(h->N, h->W, h->Mp))
+        return false;
+    const std::string want = stage_kernel_request();
+    return want == "pipelined" || (want.empty() && 2 * h->ntiles > device_cus(h));
+}
+
+void choose_stage_kernel(heom_handle* h)
+{
+    const std::string want = stage_kernel_request();
+    const int cus = device_cus(h);
     const bool small = 2 * h->ntiles <= cus;
     h->tile_split = 1;
-    if (h->N <= kMaxStates && (want == "split" || (want.empty() && small)))
+    if (h->N <= kMaxStates && !h->paired && (want == "split" || (want.empty() && small)))
         for (int sp : {4, 2})
             if (h->tile_split == 1 && tile_split_supported(h->N, sp))
                 h->tile_split = sp;
-    h->pipelined = want != "tile" && want != "split" && h->tile_split == 1 &&
-                   pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
+    // complex128: the pipelined kernel where it applies; complex64: the layout decides
+    // (its pipelined kernel on paired planes, the tile kernel on one plane per element)
+    h->pipelined = h->fp32() ? h->paired
+                             : want != "tile" && want != "split" && h->tile_split == 1 &&
+                                   pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
     if (h->pipelined) {
         const int wgs = std::max(1, std::min(cus, h->ntiles));
         h->pipe_wgs = (wgs + kNumXcd - 1) / kNumXcd * kNumXcd;
@@ -1136,12 +1243,14 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
     h->norms.alloc(h->M);
     h->mx.alloc((size_t)count * h->nel);
     h->small.alloc((size_t)count * h->nel);
-    const size_t plane = (size_t)N * N * Mp * h->cbytes();
+    build_coefficients(h);
+    h->paired = use_paired_layout(h);
+    const size_t plane = h->paired ? (size_t)pair_planes(N) * Mp * 2 * sizeof(float2)
+                                   : (size_t)N * N * Mp * h->cbytes();
     for (auto* b : {&h->y, &h->pbuf, &h->ya, &h->yb}) {
         b->alloc(plane);
         HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
     }
-    build_coefficients(h);
     // H of each instance, padded to N x N
     std::vector<double> H((size_t)count * h->nel * 2, 0.0);
     for (int k = 0; k < count; ++k)
@@ -1165,7 +1274,7 @@ void upload_state(heom_handle* h, const double* state)
     HIPC(hipMemcpyAsync(stage.p, state, stage.bytes(), hipMemcpyHostToDevice, h->stream));
     if (h->fp32()) {
         hipLaunchKernelGGL(aos_to_soa<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, stage.p,
-                           h->y.as<float2>(), h->q2u.p, h->Mp, h->nel);
+                           h->y.as<float2>(), h->q2u.p, h->Mp, h->N, h->paired);
         std::vector<double2> rows((size_t)h->S * h->nel);
         for (int s = 0; s < h->S; ++s)
             std::memcpy(&rows[(size_t)s * h->nel], state + (size_t)h->sp_uid[s] * h->nel * 2,
@@ -1176,7 +1285,7 @@ void upload_state(heom_handle* h, const double* state)
         HIPC(hipStreamSynchronize(h->stream));
     } else {
         hipLaunchKernelGGL(aos_to_soa<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, stage.p,
-                           h->y.as<double2>(), h->q2u.p, h->Mp, h->nel);
+                           h->y.as<double2>(), h->q2u.p, h->Mp, h->N, false);
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
     }
@@ -1192,10 +1301,10 @@ void download_state(heom_handle* h, const unsigned char* src, const double2* row
         HIPC(hipMemsetAsync(stage.p, 0, stage.bytes(), h->stream));
     if (h->fp32())
         hipLaunchKernelGGL(soa_to_aos<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
-                           reinterpret_cast<const float2*>(src), stage.p, h->q2u.p, h->Mp, h->nel);
+                           reinterpret_cast<const float2*>(src), stage.p, h->q2u.p, h->Mp, h->N, h->paired);
     else
         hipLaunchKernelGGL(soa_to_aos<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
-                           reinterpret_cast<const double2*>(src), stage.p, h->q2u.p, h->Mp, h->nel);
+                           reinterpret_cast<const double2*>(src), stage.p, h->q2u.p, h->Mp, h->N, false);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(state, stage.p, stage.bytes(), hipMemcpyDeviceToHost, h->stream));
     std::vector<double2> sh;
@@ -1219,7 +1328,7 @@ void gather_tops(heom_handle* h, double2* dst)
                            h->sp_top_d.p, h->count, h->nel);
     else
         hipLaunchKernelGGL(gather_ados<double2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream,
-                           h->y.as<double2>(), dst, h->q_top.p, h->count, h->Mp, h->nel);
+                           h->y.as<double2>(), dst, h->q_top.p, h->count, h->Mp, h->N, false);
     HIPC(hipGetLastError());
 }
 
@@ -1245,14 +1354,14 @@ void mmult_left_planes(heom_handle* h)
         return;
     }
     switch (h->N) {
-    case 1: hipLaunchKernelGGL((mmult_left_kernel<C, 1>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 2: hipLaunchKernelGGL((mmult_left_kernel<C, 2>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 3: hipLaunchKernelGGL((mmult_left_kernel<C, 3>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 4: hipLaunchKernelGGL((mmult_left_kernel<C, 4>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 5: hipLaunchKernelGGL((mmult_left_kernel<C, 5>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 6: hipLaunchKernelGGL((mmult_left_kernel<C, 6>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    case 7: hipLaunchKernelGGL((mmult_left_kernel<C, 7>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
-    default: hipLaunchKernelGGL((mmult_left_kernel<C, 8>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp); break;
+    case 1: hipLaunchKernelGGL((mmult_left_kernel<C, 1>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 2: hipLaunchKernelGGL((mmult_left_kernel<C, 2>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 3: hipLaunchKernelGGL((mmult_left_kernel<C, 3>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 4: hipLaunchKernelGGL((mmult_left_kernel<C, 4>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 5: hipLaunchKernelGGL((mmult_left_kernel<C, 5>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 6: hipLaunchKernelGGL((mmult_left_kernel<C, 6>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    case 7: hipLaunchKernelGGL((mmult_left_kernel<C, 7>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
+    default: hipLaunchKernelGGL((mmult_left_kernel<C, 8>), grid, block, 0, h->stream, y, h->mx.p, ti, h->Mp, h->paired); break;
     }
     HIPC(hipGetLastError());
 }
@@ -1282,12 +1391,16 @@ std::vector<StageConfig> stage_candidates(heom_handle* h)
         for (int T = wide_max_ados(h->N, (int)h->cbytes()); T >= 1; T /= 2)
             cands.push_back({HEOM_STAGE_WIDE, 1, 0, T});
     } else {
-        for (int c : chunks(std::max(1, nt / kNumXcd)))
-            cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
-        for (int sp : {2, 4})  // tiles split over workgroups, at the default chunk
-            if (tile_split_supported(h->N, sp))
-                cands.push_back({HEOM_STAGE_TILE, std::max(1, nt / kNumXcd), 0, kTile, sp});
-        if (pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes())) {
+        // complex64 planes: the layout fixes the kernel family (paired: pipelined only)
+        const bool pipe_ok = h->fp32() ? h->paired : pipe_supported(h->N, h->W, h->Mp, (int)h->cbytes());
+        if (!h->paired) {
+            for (int c : chunks(std::max(1, nt / kNumXcd)))
+                cands.push_back({HEOM_STAGE_TILE, c, 0, kTile});
+            for (int sp : {2, 4})  // tiles split over workgroups, at the default chunk
+                if (tile_split_supported(h->N, sp))
+                    cands.push_back({HEOM_STAGE_TILE, std::max(1, nt / kNumXcd), 0, kTile, sp});
+        }
+        if (pipe_ok) {
             int cus = 0;
             HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
             const int wgs = (std::max(1, std::min(cus, nt)) + kNumXcd - 1) / kNumXcd * kNumXcd;
@@ -1513,12 +1626,12 @@ int heom_set_top(heom_handle* h, const double* top)
         HIPC(hipMemcpyAsync(h->small.p, top, n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
         if (h->fp32()) {
             hipLaunchKernelGGL(scatter_ados<float2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
-                               h->y.as<float2>(), h->q_top.p, h->count, h->Mp, h->nel);
+                               h->y.as<float2>(), h->q_top.p, h->count, h->Mp, h->N, h->paired);
             hipLaunchKernelGGL(scatter_rows, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
                                h->y64.p, h->sp_top_d.p, h->count, h->nel);
         } else {
             hipLaunchKernelGGL(scatter_ados<double2>, dim3(blocks_for(n, 256)), dim3(256), 0, h->stream, h->small.p,
-                               h->y.as<double2>(), h->q_top.p, h->count, h->Mp, h->nel);
+                               h->y.as<double2>(), h->q_top.p, h->count, h->Mp, h->N, false);
         }
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
@@ -1615,10 +1728,10 @@ int heom_get_ados(heom_handle* h, int32_t instance, const int32_t* tuples, int64
             const dim3 grid(blocks_for((int64_t)cnt * nel, 256)), block(256);
             if (h->fp32())
                 hipLaunchKernelGGL(gather_ados<float2>, grid, block, 0, h->stream, h->y.as<float2>(), buf.p, qs.p, cnt,
-                                   h->Mp, nel);
+                                   h->Mp, h->N, h->paired);
             else
                 hipLaunchKernelGGL(gather_ados<double2>, grid, block, 0, h->stream, h->y.as<double2>(), buf.p, qs.p,
-                                   cnt, h->Mp, nel);
+                                   cnt, h->Mp, h->N, false);
             HIPC(hipGetLastError());
             host.resize((size_t)cnt * nel);
             HIPC(hipMemcpyAsync(host.data(), buf.p, host.size() * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
@@ -1656,10 +1769,10 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
             HIPC(hipMemsetAsync(h->norms.p, 0, h->norms.bytes(), h->stream));
         if (h->fp32())
             hipLaunchKernelGGL(hierarchy_norm_kernel<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
-                               h->y.as<float2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p, h->mask.p);
+                               h->y.as<float2>(), h->q2u.p, h->Mp, h->N, h->paired, threshold, h->norms.p, h->mask.p);
         else
             hipLaunchKernelGGL(hierarchy_norm_kernel<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0,
-                               h->stream, h->y.as<double2>(), h->q2u.p, h->Mp, h->nel, threshold, h->norms.p,
+                               h->stream, h->y.as<double2>(), h->q2u.p, h->Mp, h->N, false, threshold, h->norms.p,
                                h->mask.p);
         HIPC(hipGetLastError());
         if (threshold >= 0.0) {

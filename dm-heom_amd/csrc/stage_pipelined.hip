@@ -18,10 +18,8 @@
 // is unrolled by two); that lets the compiler see that loads into one buffer do not
 // alias reads of the other, so it does not wait for the prefetch before computing.
 // LDS element slots are laid out per matrix row with the row padded to the wavefronts'
-// 4-column blocks (pipe_row_slots); with complex64 state one 16-byte-per-lane load
-// fills two adjacent slots, and the padding keeps it inside the loading wavefront's own
-// slots.  Compiled twice: complex128/FP64 here, complex64/FP32 with
-// HEOM_STAGE_PRECISION defined (stage_pipelined_f32.hip).
+// 4-column blocks (pipe_row_slots).  Instantiated for complex128 state; complex64 state
+// runs the paired kernel (stage_pair.hip), whose 16-byte gathers serve two elements.
 #include "heom_device.hpp"
 
 namespace heom_hip {
@@ -51,41 +49,23 @@ __device__ __forceinline__ void prefetch_tile(const StageArgsT<R>& a, int tile, 
 {
     using S = Shape<R, N>;
     const int64_t mp = a.mp;
-    if (sizeof(c2<R>) == 16) {
-        const uint32_t q = (uint32_t)tile * kTile + lane;
+    static_assert(sizeof(c2<R>) == 16, "complex128 state: one 16-byte element per lane");
+    const uint32_t q = (uint32_t)tile * kTile + lane;
 #pragma unroll
-        for (int c = 0; c < S::CW; ++c) {
-            const int j = jb + c;
-            if (j < N)
-                __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)(i * N + j) * mp + q),
-                                                 HEOM_LDS_PTR(sig + (i * S::RS + j) * kTile), 16, 0, 0);
-        }
-    } else {
-        // two elements per load: lanes 0-31 element j, lanes 32-63 element j+1 (j again
-        // when j+1 is past the row; it lands in a padding slot), two ADOs per lane
-        const uint32_t q2 = (uint32_t)tile * kTile + (lane & 31) * 2;
-#pragma unroll
-        for (int c = 0; c < S::CW; c += 2) {
-            const int j = jb + c;
-            if (j < N) {
-                const int jj = (lane >= 32 && j + 1 < N) ? j + 1 : j;
-                __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)(i * N + jj) * mp + q2),
-                                                 HEOM_LDS_PTR(sig + (i * S::RS + j) * kTile), 16, 0, 0);
-            }
-        }
+    for (int c = 0; c < S::CW; ++c) {
+        const int j = jb + c;
+        if (j < N)
+            __builtin_amdgcn_global_load_lds((const void*)(a.yin + (int64_t)(i * N + j) * mp + q),
+                                             HEOM_LDS_PTR(sig + (i * S::RS + j) * kTile), 16, 0, 0);
     }
-    // table rows in pairs: lanes 0-31 load row 2u, lanes 32-63 row 2u+1, two ADOs per
-    // lane; byte offsets fit in 32 bits (pipe_supported)
-    const char* nb = (const char*)a.nb;
-    const uint32_t lane_off = (uint32_t)(((lane >> 5) * mp + (lane & 31) * 2) * sizeof(int2));
+    // table rows in pairs: lanes 0-31 load row 2u, lanes 32-63 row 2u+1, two ADOs per lane
+    const int2* nb = a.nb + (int64_t)tile * kTile + (lane >> 5) * mp + (lane & 31) * 2;
 #pragma unroll
     for (int r = 0; r < (S::TC / 2 + S::NWAVE - 1) / S::NWAVE; ++r) {
         const int u = w + r * S::NWAVE;
-        if (u < a.table_units) {
-            const uint32_t off = (uint32_t)((2 * u * mp + (int64_t)tile * kTile) * sizeof(int2)) + lane_off;
-            __builtin_amdgcn_global_load_lds((const void*)(nb + off), HEOM_LDS_PTR(tab + 2 * u * kTile), 16, 0,
+        if (u < a.table_units)
+            __builtin_amdgcn_global_load_lds((const void*)(nb + 2 * u * mp), HEOM_LDS_PTR(tab + 2 * u * kTile), 16, 0,
                                              kTableCachePolicy);
-        }
     }
 }
 
@@ -134,14 +114,13 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
     // and are not stored: every wavefront runs the same straight-line code, which keeps
     // the per-column arrays in registers.
     auto col = [&](int c) { return min(jb + c, N - 1); };
-    // Neighbour gathers as raw buffer loads over the stage input: an edge to the zero
-    // slot (missing or masked neighbour; most minus lanes of the deepest tier) gets an offset
-    // past the buffer's size, which returns zero without a memory request.
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)a.yin, (short)0, (int)(N * N * mp * (int64_t)sizeof(C)), 0x00020000);
-    const uint32_t zero = (uint32_t)a.zero_slot;
-    auto gather = [&](uint32_t off_e, uint32_t idx) {
-        const uint32_t vo = idx == zero ? 0x80000000u : off_e + idx * (uint32_t)sizeof(C);
+    // Neighbour gathers as raw buffer loads, one buffer resource per element plane whose
+    // range ends at the zero slot (the first slot after every ADO): an edge to the zero
+    // slot (missing or masked neighbour; most minus lanes of the deepest tier) is out of
+    // range, which returns zero without a memory request.
+    const int nrec = a.zero_slot * (int)sizeof(C);
+    auto gather = [&](__amdgpu_buffer_rsrc_t rsrc, uint32_t idx) {
+        const uint32_t vo = idx * (uint32_t)sizeof(C);
         if constexpr (sizeof(C) == 16) {
             const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, vo, 0, 0);
             return mk2<R>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
@@ -152,7 +131,8 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
     };
     auto load = [&](int c) {
         const int e = i * N + col(c);
-        const uint32_t off_e = (uint32_t)(e * mp * (int64_t)sizeof(C));
+        const __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(a.yin + e * mp), (short)0, nrec, 0x00020000);
         const TermT<R>* te = opaque(terms) + e * NT;
 #pragma unroll
         for (int k = 0; k < NT; ++k)
@@ -160,8 +140,8 @@ __device__ __forceinline__ void compute_tile(const StageArgsT<R>& a, const c2<R>
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             if constexpr (PLUS)
-                L[c].xp[k] = gather(off_e, (uint32_t)L[c].nbv[k].x >> 6);
-            L[c].xm[k] = gather(off_e, (uint32_t)L[c].nbv[k].y);
+                L[c].xp[k] = gather(rsrc, (uint32_t)L[c].nbv[k].x >> 6);
+            L[c].xm[k] = gather(rsrc, (uint32_t)L[c].nbv[k].y);
         }
     };
 
@@ -360,18 +340,15 @@ void launch_any(int N, int nterm_slots, bool real_h, int stage, const StageArgsT
 
 }  // namespace
 
-#ifndef HEOM_STAGE_PRECISION
 bool pipe_supported(int N, int W, int64_t mp, int complex_bytes)
 {
-    // wide enough workgroups to carry a CU on their own (with complex64 state only
-    // N = 8, whose column blocks are full: at N = 6, 7 that kernel spills registers), the
-    // table slice fits the LDS buffer, and table byte offsets fit in 32 bits
-    const bool shape = complex_bytes == (int)sizeof(double2) ? N >= 6 && N <= kMaxStates : N == kMaxStates;
-    // (and the state below 2 GiB: the gathers are buffer loads whose size and out-of-range
-    // marker are 32-bit)
-    return shape && ((W + 2) & ~1) <= pipe_table_capacity(N, complex_bytes) &&
-           (int64_t)(W + 2) * mp * (int64_t)sizeof(int2) < (int64_t(1) << 32) &&
-           (int64_t)N * N * mp * complex_bytes < (int64_t(1) << 31);
+    // complex128 state (complex64 state runs the paired kernel, stage_pair.hip); wide
+    // enough workgroups to carry a CU on their own and a table slice that fits the LDS
+    // buffer.  Each element plane has its own buffer resource (at most 2^26 slots of 16
+    // bytes), so the state size is not limited beyond that.
+    (void)mp;
+    const bool shape = complex_bytes == (int)sizeof(double2) && N >= 6 && N <= kMaxStates;
+    return shape && ((W + 2) & ~1) <= pipe_table_capacity(N, complex_bytes);
 }
 
 void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
@@ -379,12 +356,5 @@ void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, cons
 {
     launch_any<double>(N, nterm_slots, real_h, stage, a, el, workgroups, s);
 }
-#else
-void launch_stage_pipelined(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<float>& a,
-                            const ElementArgsT<float>& el, int workgroups, hipStream_t s)
-{
-    launch_any<float>(N, nterm_slots, real_h, stage, a, el, workgroups, s);
-}
-#endif
 
 }  // namespace heom_hip

@@ -97,3 +97,39 @@ def test_fp32_batch_matches_single(oracle_mod):
             h.set_top(top)
             rec1 = h.propagate(1e-15, 40, 10)
         assert np.array_equal(rec[:, k], rec1), k
+
+
+# (states, modes per bath, complex Hamiltonian): site-local baths, so a paired plane
+# couples to 2K modes (term slots 2, 4, 8); every N of the paired kernel, both
+# Hamiltonian forms
+PAIR_CASES = [(4, 1, False), (5, 2, True), (6, 4, False), (7, 1, True), (7, 3, False), (8, 2, False),
+              (8, 4, True)]
+
+
+@pytest.mark.parametrize("N,K,complex_h", PAIR_CASES)
+def test_fp32_paired_planes(oracle_mod, stage_kernel, N, K, complex_h):
+    """The paired complex64 layout (elements (i, j) and (j, i) in one 16-byte plane slot)
+    runs whenever the pipelined kernel is asked for with 4 to 8 states; one RHS and ten
+    RK4 steps against the FP64 oracle."""
+    from test_gpu_parity import make_instance
+
+    inst = make_instance(N, N, K, 3, [[i] for i in range(N)], seed=10 * N + K, complex_h=complex_h)
+    y = random_state(inst, N + K)
+    t = tier01(inst)
+    y32 = as_complex(y.astype(np.float32).astype(np.float64), inst)
+    y32[t] = as_complex(y, inst)[t]
+    y_in = np.ascontiguousarray(np.stack([y32.real, y32.imag], -1).reshape(-1))
+    ref = inst.rhs(y_in)
+    dt, steps = 1e-15, 10
+    ys = y_in.copy()
+    for _ in range(steps):
+        inst.rk4_step(ys, dt, np.empty(5 * inst.state_size))
+    with heom_amd.Handle.from_instance(inst, precision=FP32) as h:
+        assert h.stage_kernel() == ("pipelined" if stage_kernel == "pipelined" else "tile")
+        h.set_state(y)
+        got = h.rhs()
+        h.step_rk4(dt, steps)
+        stepped = h.get_state()
+    e1, e2 = max_abs_rel(got, ref), max_abs_rel(stepped, ys)
+    print(f"N={N} K={K} complex_h={complex_h}: rhs {e1:.1e}, {steps} steps {e2:.1e}", flush=True)
+    assert e1 < 1e-5 and e2 < 1e-5
