@@ -431,6 +431,14 @@ def pcie_inclusive(h, dt, steps, M, wall_steps) -> dict:
             "note": f"{steps} steps with the {gb:.2f} GB state uploaded before and downloaded after them"}
 
 
+def stage_kernel_name(h, fp32: bool) -> str:
+    """The stage kernel a handle runs, by its template name."""
+    kind = h.stage_kernel()
+    if kind == "pipelined":
+        return "rk4_stage_pair" if fp32 else "rk4_stage_pipe"
+    return "rk4_stage_wide" if kind == "wide" else "rk4_stage"
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -522,12 +530,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_note": None if traffic is None else (
-                "bytes per launch between the L2 and the memory fabric (rocprofv3: FETCH_SIZE x 2 + WRITE_SIZE; "
-                "every L2 read request is 128 B on gfx950, TCC_EA0_RDREQ x 128 B equals FETCH_SIZE x 2); it "
+                "bytes per launch between the L2 and the memory fabric (rocprofv3: reads as "
+                f"{traffic_rec.get('read_counter', 'FETCH_SIZE x 2 KiB')}, writes as WRITE_SIZE; every L2 read "
+                "request is 128 B on gfx950, so TCC_EA0_RDREQ x 128 B and FETCH_SIZE x 2 KiB count the same bytes); it "
                 "includes hits in the 256 MB last-level cache (MALL), which no L2-side counter separates from "
                 f"DRAM (TCC_EA0_RDREQ_DRAM equals TCC_EA0_RDREQ); measured on build {traffic_rec.get('build')} with "
                 f"{traffic_rec.get('xcd_chunk')} tiles per XCD chunk, {traffic_rec.get('source')}"),
-            "kernel": "rk4_stage (RHS + RK4 stage epilogue), 4 launches per step",
+            "kernel": f"{stage_kernel_name(h, fp32)} (RHS + RK4 stage epilogue), 4 launches per step",
             "alg_bytes_per_launch": alg_bytes_per_launch,
             "avg_launch_ms": per_launch_s * 1e3,
             "measured_copy_GBs": copy_gbs,
