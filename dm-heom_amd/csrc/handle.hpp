@@ -164,6 +164,11 @@ struct heom_handle {
     bool uniform = true;               // one bath class and every instance of N states
 
     DevBuf<unsigned char> y, pbuf, ya, yb;  // planes, complex128 or complex64
+    // the per-evaluation interface (evaluate.hip): state buffers 4.. added by the caller,
+    // the planes of f for the accumulating forms, and the pre-evaluate actions
+    std::vector<std::unique_ptr<DevBuf<unsigned char>>> extra;
+    DevBuf<unsigned char> fscratch;
+    std::vector<std::pair<heom_pre_evaluate_fn, void*>> pre_evaluate;
     DevBuf<int2> nb_raw, nb;                // packed neighbour table, unmasked and mask-folded
     DevBuf<int32_t> q2u, tile_inst;
     DevBuf<int32_t> tile_plus;              // [ntiles] 1 if an ADO of the tile has a plus edge
@@ -250,6 +255,14 @@ void gather_tops(heom_handle* h, double2* dst);
 void observe_launch(heom_handle* h, const double2* ops, double* pops, double2* traces);
 // planes in the internal layout -> reference layout (state [M*N*N*2], host)
 void download_state(heom_handle* h, const unsigned char* src, const double2* rows, double* state);
+// reference layout -> the state planes, or (FP64) the planes dst
+void upload_state(heom_handle* h, const double* state, unsigned char* dst = nullptr);
+// drops the captured RK4 step (its buffer addresses or coefficients changed)
+void drop_graph(heom_handle* h);
+// evaluate.hip: state buffer b (0 the state, 1-3 the RK4 work buffers, 4.. added ones)
+DevBuf<unsigned char>& state_buffer(heom_handle* h, int b);
+// evaluate.hip: the added buffers at the current layout size, zeroed (after a re-layout)
+void reset_extra_buffers(heom_handle* h);
 int count_bath(const heom_handle* h, int s, int b);
 void resolve_timing(heom_handle* h);
 // adjoint.hip: drops the recorded sweep when the operator changes

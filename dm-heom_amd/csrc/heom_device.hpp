@@ -330,9 +330,10 @@ void launch_stage_adjoint(int N, int nterm_slots, bool real_h, int stage, const 
 
 // ---- large systems (stage_wide.hip): 8 < N <= 32, one wavefront per ADO ---------------
 
-bool wide_supported(int N, int complex_bytes);
+// W: bath modes (the workgroup keeps its tile's slice of the neighbour table in LDS)
+bool wide_supported(int N, int complex_bytes, int W);
 // the largest number of ADOs per workgroup (a power of two <= 8) whose tiles fit in LDS
-int wide_max_ados(int N, int complex_bytes);
+int wide_max_ados(int N, int complex_bytes, int W);
 // ados: ADOs per workgroup, 0 = wide_max_ados
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                        const ElementArgsT<double>& el, int ados, hipStream_t s);

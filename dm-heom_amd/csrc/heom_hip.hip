@@ -894,8 +894,9 @@ void validate(const heom_desc* d)
         raise(HEOM_E_INVALID, "desc is NULL");
     if (d->states < 1)
         raise(HEOM_E_INVALID, "states must be > 0");
-    if (d->states > kMaxStates && !wide_supported(d->states, d->precision == HEOM_PRECISION_FP32 ? 8 : 16))
-        raise(HEOM_E_UNSUPPORTED, "states > 32 not supported by this build");
+    if (d->states > kMaxStates &&
+        !wide_supported(d->states, d->precision == HEOM_PRECISION_FP32 ? 8 : 16, std::max(d->width, 0)))
+        raise(HEOM_E_UNSUPPORTED, "states > 32 (or too many modes for the large-system kernel) not supported by this build");
     if (d->matsubaras < 1 || d->baths < 1 || d->baths_max_per_state < 1)
         raise(HEOM_E_INVALID, "matsubaras, baths and baths_max_per_state must be > 0");
     if (d->width != d->baths * d->matsubaras)
@@ -1098,8 +1099,8 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
             h->classes.push_back(std::move(bc));
     }
     h->uniform = h->classes.size() == 1;
-    if (N > kMaxStates && !wide_supported(N, d[0].precision == HEOM_PRECISION_FP32 ? 8 : 16))
-        raise(HEOM_E_UNSUPPORTED, "states > 32 not supported by this build");
+    if (N > kMaxStates && !wide_supported(N, d[0].precision == HEOM_PRECISION_FP32 ? 8 : 16, W))
+        raise(HEOM_E_UNSUPPORTED, "states > 32 (or too many modes for the large-system kernel) not supported by this build");
     const BathClass& c0 = h->classes[0];
     h->N = N;
     h->K = c0.K;
@@ -1267,7 +1268,7 @@ void create_batch(heom_handle* h, const heom_desc* d, int count)
 }
 
 // state in reference layout (AoS, all instances) -> planes (and FP64 tier shadows)
-void upload_state(heom_handle* h, const double* state)
+void upload_state(heom_handle* h, const double* state, unsigned char* dst)
 {
     DevBuf<double2> stage;
     stage.alloc((size_t)h->M * h->nel);
@@ -1285,7 +1286,7 @@ void upload_state(heom_handle* h, const double* state)
         HIPC(hipStreamSynchronize(h->stream));
     } else {
         hipLaunchKernelGGL(aos_to_soa<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream, stage.p,
-                           h->y.as<double2>(), h->q2u.p, h->Mp, h->N, false);
+                           reinterpret_cast<double2*>(dst ? dst : h->y.p), h->q2u.p, h->Mp, h->N, false);
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
     }
@@ -1388,7 +1389,7 @@ std::vector<StageConfig> stage_candidates(heom_handle* h)
         return out;
     };
     if (h->N > kMaxStates) {
-        for (int T = wide_max_ados(h->N, (int)h->cbytes()); T >= 1; T /= 2)
+        for (int T = wide_max_ados(h->N, (int)h->cbytes(), h->W); T >= 1; T /= 2)
             cands.push_back({HEOM_STAGE_WIDE, 1, 0, T});
     } else {
         // complex64 planes: the layout fixes the kernel family (paired: pipelined only)
@@ -2195,6 +2196,7 @@ int heom_compact(heom_handle* h, int64_t* active)
         h->ntiles = ntiles;
         h->zero_slot = zero;
         h->active = kept;
+        reset_extra_buffers(h);  // added state buffers follow the new layout, zeroed
         build_tile_plus(h, nb_n);
         upload(h->pfss, pfss_n, h->stream);
         h->inst_q = new_q;

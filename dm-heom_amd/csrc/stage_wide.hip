@@ -31,13 +31,26 @@ __host__ __device__ constexpr int wide_row(int N) { return N + 1; }
 // block's accumulators, operands and in-flight loads)
 constexpr int kWideThreads = 512;
 
-// ADOs per workgroup for N states with `cbytes` per complex element, 0 if none fits:
-// the LDS holds H and the tile (later overwritten by the commutator-plus-same-term values)
-int wide_tile(int N, int cbytes)
+// (ADO, element) pairs a thread carries through one pass of phase C
+constexpr int kWidePairsPerPass = 2;
+
+// table rows a workgroup copies to LDS for W modes: rows 0..W (mode W is the zero row
+// that padding terms name), rounded up to an even count
+__host__ __device__ constexpr int wide_table_rows(int W) { return (W + 2) & ~1; }
+
+// LDS bytes of a workgroup of T ADOs: H, the tile (later overwritten by the commutator-
+// plus-same-term values) and the tile's slice of the neighbour table
+__host__ __device__ constexpr int wide_lds_bytes(int N, int cbytes, int W, int T)
 {
-    const int per = N * wide_row(N) * cbytes;
+    return (T + 1) * N * wide_row(N) * cbytes + wide_table_rows(W) * T * (int)sizeof(int2);
+}
+
+// ADOs per workgroup for N states with `cbytes` per complex element and W modes, 0 if
+// none fits
+int wide_tile(int N, int cbytes, int W)
+{
     for (int T = kWideThreads / 64; T >= 1; T /= 2)
-        if ((T + 1) * per <= kWideLdsBytes)
+        if (wide_lds_bytes(N, cbytes, W, T) <= kWideLdsBytes)
             return T;
     return 0;
 }
@@ -55,7 +68,7 @@ int wide_tile(int N, int cbytes)
 //      the RK4 epilogue with the own element re-read from global memory (still in L2),
 //      so that loads of the stage operand and all stores move T consecutive ADOs of
 //      one plane together.  LDS: H and one tile, so that several workgroups share a CU.
-template <typename R, int B, int NT, int STAGE, bool REALH, int LDSB>
+template <typename R, int B, int NT, int STAGE, bool REALH, int LDSB, int PP = kWidePairsPerPass>
 __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, const c2<R>* __restrict__ hs_all,
                                                                const c2<R>* __restrict__ gam,
                                                                const TermT<R>* __restrict__ terms, int N, int T)
@@ -66,6 +79,8 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
     const int nel = N * N, NP = wide_row(N), mat = N * NP;
     C* hl = lds;         // [N][NP] H / hbar
     C* sig = lds + mat;  // [T][N][NP] the tile, then commutator + same term
+    int2* tabl = reinterpret_cast<int2*>(sig + T * mat);  // [rows][T] the tile's table slice
+    const int trows = 2 * a.table_units;
     // index arithmetic without runtime divisions: T is a power of two, and
     // e / N = (e * ninv) >> 20 for every e < 8256 and N <= 32
     const int tsh = __builtin_ctz((unsigned)T);
@@ -89,6 +104,8 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
         const int ad = idx & (T - 1), e = idx >> tsh, r = div_n(e);
         sig[ad * mat + r * NP + e - r * N] = a.yin[(int64_t)e * mp + q0 + ad];
     }
+    for (int idx = threadIdx.x; idx < trows * T; idx += blockDim.x)
+        tabl[idx] = a.nb[(int64_t)(idx >> tsh) * mp + q0 + (idx & (T - 1))];
     __syncthreads();
 
     // phase B
@@ -153,35 +170,36 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
     }
     __syncthreads();
 
-    // phase C, two (ADO, element) pairs per thread and pass so that their loads overlap;
+    // phase C, PP (ADO, element) pairs per thread and pass so that their loads overlap
+    // (table entries from LDS, then the gathers, stage operands and own values together);
     // PLUS = false (no ADO of the 64-ADO tile that holds these T has a plus neighbour)
     // leaves out the plus gathers
     auto phase_c = [&](auto plus_tag) {
         constexpr bool PLUS = decltype(plus_tag)::value;
         const int total = T * nel, stride = blockDim.x;
-        for (int base = threadIdx.x; base < total; base += 2 * stride) {
-            int2 nbv[2][NT];
-            C ep[2], own[2], xp[2][NT], xm[2][NT];
-            int64_t o[2];
-            int e2[2], lo[2];
-            bool live[2];
+        for (int base = threadIdx.x; base < total; base += PP * stride) {
+            int2 nbv[PP][NT];
+            C ep[PP], own[PP], xp[PP][NT], xm[PP][NT];
+            int64_t o[PP];
+            int e2[PP], lo[PP];
+            bool live[PP];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < PP; ++u) {
                 const int idx = base + u * stride;
                 live[u] = idx < total;
-                const int id = live[u] ? idx : base;  // a dead second pair repeats the first's loads
+                const int id = live[u] ? idx : base;  // a dead pair repeats the first pair's loads
                 const int ad = id & (T - 1), e = id >> tsh, r = div_n(e);
                 e2[u] = e;
                 lo[u] = ad * mat + r * NP + e - r * N;
                 o[u] = (int64_t)e * mp + q0 + ad;
 #pragma unroll
                 for (int k = 0; k < NT; ++k)
-                    nbv[u][k] = a.nb[(int64_t)terms[e * NT + k].mode * mp + q0 + ad];
+                    nbv[u][k] = tabl[terms[e * NT + k].mode * T + ad];
                 ep[u] = stage_operand<STAGE, R>(a, o[u]);
                 own[u] = a.yin[o[u]];
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < PP; ++u) {
                 const C* plane = a.yin + (int64_t)e2[u] * mp;
 #pragma unroll
                 for (int k = 0; k < NT; ++k) {
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < PP; ++u) {
                 if (!live[u])
                     continue;
                 C f = sig[lo[u]];
@@ -236,7 +254,7 @@ void launch_wl(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& 
 template <typename R, int B, int NT, bool REALH>
 void launch_wb(int N, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int T, hipStream_t st)
 {
-    if ((int64_t)(T + 1) * N * wide_row(N) * (int64_t)sizeof(c2<R>) <= kWideLdsSmallBytes)
+    if (wide_lds_bytes(N, (int)sizeof(c2<R>), 2 * a.table_units - 2, T) <= kWideLdsSmallBytes)
         launch_wl<R, B, NT, REALH, kWideLdsSmallBytes>(N, stage, a, el, T, st);
     else
         launch_wl<R, B, NT, REALH, kWideLdsBytes>(N, stage, a, el, T, st);
@@ -255,7 +273,7 @@ void launch_wn(int N, int nt, int stage, const StageArgsT<R>& a, const ElementAr
 template <typename R, bool REALH>
 void launch_w(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArgsT<R>& el, int ados, hipStream_t st)
 {
-    const int fit = wide_tile(N, (int)sizeof(c2<R>));
+    const int fit = wide_tile(N, (int)sizeof(c2<R>), 2 * a.table_units - 2);
     const int T = ados > 0 && ados <= fit && kTile % ados == 0 ? ados : fit;
     const int B = (N + 7) / 8;
     if (B <= 2)
@@ -268,12 +286,12 @@ void launch_w(int N, int nt, int stage, const StageArgsT<R>& a, const ElementArg
 
 }  // namespace
 
-bool wide_supported(int N, int complex_bytes)
+bool wide_supported(int N, int complex_bytes, int W)
 {
-    return N > kMaxStates && N <= kMaxWideStates && wide_tile(N, complex_bytes) > 0;
+    return N > kMaxStates && N <= kMaxWideStates && wide_tile(N, complex_bytes, W) > 0;
 }
 
-int wide_max_ados(int N, int complex_bytes) { return wide_tile(N, complex_bytes); }
+int wide_max_ados(int N, int complex_bytes, int W) { return wide_tile(N, complex_bytes, W); }
 
 void launch_stage_wide(int N, int nterm_slots, bool real_h, int stage, const StageArgsT<double>& a,
                        const ElementArgsT<double>& el, int ados, hipStream_t s)

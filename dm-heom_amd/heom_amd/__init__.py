@@ -41,7 +41,14 @@ EXPORTED = [
     "heom_adjoint_forward", "heom_adjoint_backward", "heom_rhs_adjoint", "heom_autotune", "heom_compact",
     "heom_propagate_timedep", "heom_create_tss", "heom_tss_step", "heom_get_ados", "heom_observe",
     "heom_create_tss_batch", "heom_set_stage_config", "heom_propagate_observe_noalloc",
+    "heom_add_pre_evaluate", "heom_clear_pre_evaluate", "heom_evaluate", "heom_lincomb", "heom_swap_buffers",
+    "heom_add_buffers", "heom_buffer_count", "heom_set_buffer", "heom_get_buffer",
 ]
+HEOM_NO_BUFFER = -1
+HEOM_MAX_BUFFERS = 16
+# int (*)(heom_handle*, double time, double step_size, int in, int out, void* user)
+PRE_EVALUATE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_void_p)
 HEOM_STAGE_TILE = 0
 HEOM_STAGE_PIPELINED = 1
 HEOM_STAGE_WIDE = 2
@@ -141,6 +148,16 @@ def load():
             "heom_create_tss_batch": [vp, ctypes.c_int32, vp, vp, vp, ctypes.POINTER(ctypes.c_void_p)],
             "heom_get_ados": [vp, ctypes.c_int32, vp, i64, vp],
             "heom_observe": [vp, vp, vp, vp],
+            "heom_add_pre_evaluate": [vp, PRE_EVALUATE_FN, vp],
+            "heom_clear_pre_evaluate": [vp],
+            "heom_evaluate": [vp, dbl, dbl, ctypes.c_int, ctypes.c_int, ctypes.c_int, dbl, ctypes.c_int, dbl,
+                              ctypes.c_int],
+            "heom_lincomb": [vp, ctypes.c_int, ctypes.c_int, vp, vp],
+            "heom_swap_buffers": [vp, ctypes.c_int, ctypes.c_int],
+            "heom_add_buffers": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+            "heom_buffer_count": [vp, ctypes.POINTER(ctypes.c_int)],
+            "heom_set_buffer": [vp, ctypes.c_int, vp],
+            "heom_get_buffer": [vp, ctypes.c_int, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -453,6 +470,66 @@ class Handle:
 
     def step_rk4(self, h: float, nsteps: int) -> None:
         _check(load().heom_step_rk4(self._h, float(h), int(nsteps)))
+
+    # ---- per-evaluation interface (heom_evaluate and the buffer algebra) ----------------
+
+    def evaluate(self, time: float, step_size: float, src: int = 0, out: int = HEOM_NO_BUFFER,
+                 out_base: int = HEOM_NO_BUFFER, out_coeff: float = 1.0, acc: int = HEOM_NO_BUFFER,
+                 acc_coeff: float = 0.0, acc_init: bool = False) -> None:
+        """f = L(buffer src) after the pre-evaluate actions; buffer out = (buffer out_base
+        or 0) + out_coeff f and buffer acc = (buffer src if acc_init else buffer acc) +
+        acc_coeff f (ode::solve, dm-heom/src/heom/ode.cpp:97-196)."""
+        _check(load().heom_evaluate(self._h, float(time), float(step_size), int(src), int(out), int(out_base),
+                                    float(out_coeff), int(acc), float(acc_coeff), 1 if acc_init else 0))
+
+    def lincomb(self, dst: int, terms) -> None:
+        """buffer dst = sum of coeff * buffer src over terms [(coeff, src), ...] (1 to 8)."""
+        terms = list(terms)
+        src = np.array([t[1] for t in terms], dtype=np.int32)
+        coeff = np.array([t[0] for t in terms], dtype=np.float64)
+        _check(load().heom_lincomb(self._h, int(dst), len(terms), src.ctypes.data, coeff.ctypes.data))
+
+    def swap_buffers(self, a: int, b: int) -> None:
+        _check(load().heom_swap_buffers(self._h, int(a), int(b)))
+
+    def add_buffers(self, n: int) -> int:
+        """Adds n zeroed state buffers; returns the number of buffers now."""
+        c = ctypes.c_int()
+        _check(load().heom_add_buffers(self._h, int(n), ctypes.byref(c)))
+        return c.value
+
+    def buffer_count(self) -> int:
+        c = ctypes.c_int()
+        _check(load().heom_buffer_count(self._h, ctypes.byref(c)))
+        return c.value
+
+    def set_buffer(self, b: int, state) -> None:
+        s = _f64(state)
+        if s.size != self.state_size:
+            raise ValueError(f"state has {s.size} values, expected {self.state_size}")
+        _check(load().heom_set_buffer(self._h, int(b), s.ctypes.data))
+
+    def get_buffer(self, b: int) -> np.ndarray:
+        out = np.empty(self.state_size)
+        _check(load().heom_get_buffer(self._h, int(b), out.ctypes.data))
+        return out
+
+    def add_pre_evaluate(self, fn) -> None:
+        """ode::add_pre_evaluate_action: fn(handle, time, step_size, src, out) runs before
+        every heom_evaluate (return None or 0; anything else fails the evaluation)."""
+        def trampoline(_h, time, step_size, src, out, _user):
+            try:
+                rc = fn(self, time, step_size, src, out)
+                return 0 if rc is None else int(rc)
+            except Exception:  # an exception cannot cross the C boundary
+                return -1
+        cb = PRE_EVALUATE_FN(trampoline)
+        self.__dict__.setdefault("_callbacks", []).append(cb)  # kept alive as long as the handle
+        _check(load().heom_add_pre_evaluate(self._h, cb, None))
+
+    def clear_pre_evaluate(self) -> None:
+        _check(load().heom_clear_pre_evaluate(self._h))
+        self._callbacks = []
 
     def propagate(self, h: float, nsteps: int, observe_steps: int) -> np.ndarray:
         """Tops every observe_steps steps: (rows, N, N); for a batch (rows, count, N, N)."""

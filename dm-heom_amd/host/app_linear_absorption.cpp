@@ -45,6 +45,10 @@ int main(int argc, char** argv)
         std::vector<heom::complex_t> acc((std::size_t)iterations + 1, heom::complex_t(0.0, 0.0));
         const std::size_t ncomp = cfg.dipole_tensor_prefactors().size();
         double device_s = 0.0;
+        heom::time_stats steps_total, ode_total;
+        std::int64_t device_bytes = 0;
+        double hierarchy_bytes = 0.0, instance_bytes = 0.0;
+        std::string stepper;
         for (std::size_t t = 0; t < ncomp; ++t) {
             heom::instance inst(cfg, heom::sites_to_states_mode::with_ground_state, graph);
             inst.set_hierarchy_top();
@@ -72,7 +76,14 @@ int main(int argc, char** argv)
                 }
                 acc[(std::size_t)r] += tr;
             }
-            device_s += solver.timing().rk4_seconds;
+            const heom_timing_info ti = solver.timing();
+            device_s += ti.rk4_seconds;
+            steps_total.add(solver.step_stats().sum, solver.step_stats().count);
+            ode_total.add(ti.rk4_seconds, ti.rhs_launches);
+            device_bytes = solver.device_bytes();
+            stepper = solver.stepper();
+            hierarchy_bytes = (double)inst.matrices() * inst.states() * inst.states() * 16.0;
+            instance_bytes = (double)inst.allocated_byte();
             if (!cl.no_progress)
                 std::cout << "Calculation linear absorption: tensor component " << (t + 1) << " of " << ncomp
                           << std::endl;
@@ -84,6 +95,22 @@ int main(int argc, char** argv)
             out << heom::fmt_real(time) << heom::kDelimiter << heom::fmt_real(v.real()) << heom::kDelimiter
                 << heom::fmt_real(v.imag()) << '\n';
         }
+        // the runtime and memory summaries of dm-heom/src/app_linear_absorption.cpp (as in
+        // app_population_dynamics.cpp:108-127), summed over the tensor components
+        std::cout << "-------------------- Solver Runtime Summary ----------------" << std::endl;
+        heom::write_time_statistics_summary(steps_total, "solver step (" + stepper + ")", std::cout);
+        if (ode_total.sum > 0.0)
+            heom::write_time_statistics_summary(ode_total, "heom_ode kernel", std::cout);
+        std::cout << "------------------------------------------------------------" << std::endl;
+        std::cout << "-------------------- Memory Summary ------------------------" << std::endl;
+        char line[64];
+        std::snprintf(line, sizeof line, "%11.2f MiB", hierarchy_bytes / 1024.0 / 1024.0);
+        std::cout << "local hierarchy buffer size:            " << line << std::endl;
+        std::snprintf(line, sizeof line, "%11.2f MiB", instance_bytes / 1024.0 / 1024.0);
+        std::cout << "local instance size:                    " << line << std::endl;
+        std::snprintf(line, sizeof line, "%11.2f MiB", (double)device_bytes / 1024.0 / 1024.0);
+        std::cout << "device memory of the handle (HBM):      " << line << std::endl;
+        std::cout << "------------------------------------------------------------" << std::endl;
         std::cout << "solver: " << device_s << " s device time over " << ncomp << " propagations" << std::endl;
     } catch (const std::exception& e) {
         return heom::report_exception(e);

@@ -66,10 +66,16 @@ int main(int argc, char** argv)
             if (!cl.no_progress)
                 std::cout << std::endl;
         }
-        const auto t = solver.timing();
-        std::cout << "solver: " << t.rk4_steps << " RK4 steps, " << t.rk4_seconds << " s device time, "
-                  << (t.rk4_seconds > 0 ? (double)graph.nodes() * t.rk4_steps / t.rk4_seconds : 0.0)
-                  << " ADO-steps/s" << std::endl;
+        std::cout << "-------------------- Solver Runtime Summary ----------------" << std::endl;
+        heom::write_runtime_summary(solver, std::cout);
+        std::cout << "------------------------------------------------------------" << std::endl;
+        std::cout << "-------------------- Memory Summary ------------------------" << std::endl;
+        heom::write_memory_summary(inst, solver, std::cout);
+        std::cout << "------------------------------------------------------------" << std::endl;
+        const auto& st = solver.step_stats();
+        std::cout << "solver: " << st.count << " steps (" << solver.stepper() << "), "
+                  << (st.sum > 0 ? (double)graph.nodes() * (double)st.count / st.sum : 0.0) << " ADO-steps/s"
+                  << std::endl;
     } catch (const std::exception& e) {
         return heom::report_exception(e);
     }

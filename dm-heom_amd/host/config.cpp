@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "constants.hpp"
+#include "stepper.hpp"
 
 namespace heom {
 
@@ -284,8 +285,9 @@ config::config(const std::string& path, program_task task) : task_(task)
         throw config_error("'" + filtering_strategy_ + "' is not a valid filtering_strategy");
     filtering_first_layer_ = to_int(lit("filtering.first_layer"), "filtering.first_layer");
     stepper_type_ = kv.at("solver.stepper_type");
-    if (stepper_type_ != "rk_rk4")
-        throw config_error("solver.stepper_type '" + stepper_type_ + "' is not supported (rk_rk4 only)");
+    if (!find_tableau(stepper_type_))
+        throw config_error("solver.stepper_type '" + stepper_type_ + "' is not supported (one of " +
+                           stepper_names() + ")");
     step_size_ = to_real(lit("solver.step_size"), "solver.step_size");
     steps_ = to_int(lit("solver.steps"), "solver.steps");
     track_flows_ = to_bool(kv.at("solver.track_flows"), "solver.track_flows");

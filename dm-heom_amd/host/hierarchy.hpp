@@ -123,6 +123,19 @@ public:
     void write_file(const std::string& path, const double* state, bool top_only, double time) const;
     void read_state(const std::string& path, double* state, double* time) const;
 
+    // bytes of the instance's host arrays and of its graph's tables (the reference's
+    // instance::allocated_byte, as its memory summary prints it)
+    std::size_t allocated_byte() const
+    {
+        std::size_t b = 0;
+        for (const auto* v : {&hamiltonian_, &pf_same_sum_, &pf_same_, &pf_circ_, &cbk_, &cbk_sqrt_, &top_, &lambda_,
+                              &invnu_, &omega_})
+            b += v->size() * sizeof(double);
+        b += coupling_.size() * sizeof(std::int32_t);
+        b += (graph_.tuples().size() + graph_.plus().size() + graph_.minus().size()) * sizeof(std::int32_t);
+        return b;
+    }
+
 private:
     void precompute();
 

@@ -14,9 +14,11 @@
 // contract; it is "%.17g" here, tab separated.
 #pragma once
 
+#include <chrono>
 #include <complex>
 #include <cstdio>
 #include <fstream>
+#include <ostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -25,6 +27,7 @@
 #include "config.hpp"
 #include "heom_hip.h"
 #include "hierarchy.hpp"
+#include "stepper.hpp"
 #include "thermal_state.hpp"
 
 namespace heom {
@@ -35,14 +38,41 @@ inline void hip_call(int rc)
         throw std::runtime_error(std::string("heom: ") + heom_last_error());
 }
 
+// seconds summed over `count` events (bmt::statistics of the reference, reduced to what
+// its summary prints)
+struct time_stats {
+    double sum = 0.0;
+    std::int64_t count = 0;
+    void add(double seconds, std::int64_t n)
+    {
+        sum += seconds;
+        count += n;
+    }
+};
+
+inline double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 class population_dynamics_solver {
 public:
     // heom_create from the instance, initial value = the instance's top ADO
+    // solver.stepper_type: rk_rk4 runs the fused RK4 step (heom_step_rk4); any other
+    // tableau of stepper.hpp runs on the per-evaluation interface
     population_dynamics_solver(const config& cfg, instance& inst)
         : inst_(inst), step_size_(cfg.solver_step_size())
     {
         const heom_desc d = inst.desc();
         hip_call(heom_create(&d, &h_));
+        stepper_name_ = cfg.solver_stepper_type();
+        if (cfg.solver_stepper_type() != "rk_rk4") {
+            const rk_tableau* t = find_tableau(cfg.solver_stepper_type());
+            if (!t)
+                throw std::runtime_error("stepper type '" + cfg.solver_stepper_type() + "' is not one of " +
+                                         stepper_names());
+            stepper_ = std::make_unique<explicit_rk_stepper>(h_, *t);
+        }
         reset();
     }
     ~population_dynamics_solver()
@@ -65,7 +95,15 @@ public:
     // population_dynamics_solver::step_forward (population_dynamics_solver.hpp:233-265)
     void step_forward(int_t n)
     {
-        hip_call(heom_step_rk4(h_, step_size_, n));
+        const auto t0 = std::chrono::steady_clock::now();
+        if (stepper_) {
+            for (int_t k = 0; k < n; ++k)
+                stepper_->step(time_ + (double)k * step_size_, step_size_);
+        } else {
+            hip_call(heom_step_rk4(h_, step_size_, n));
+        }
+        hip_call(heom_sync(h_));
+        step_stats_.add(seconds_since(t0), n);
         step_count_ += n;
         time_ += (double)n * step_size_;
     }
@@ -76,7 +114,18 @@ public:
     {
         const int_t N = inst_.states();
         std::vector<complex_t> tops((std::size_t)((n / observe_steps + 1) * N * N));
-        hip_call(heom_propagate(h_, step_size_, n, observe_steps, reinterpret_cast<double*>(tops.data())));
+        const auto t0 = std::chrono::steady_clock::now();
+        if (stepper_) {
+            hierarchy_top(tops.data());
+            for (int_t k = 1; k <= n; ++k) {
+                stepper_->step(time_ + (double)(k - 1) * step_size_, step_size_);
+                if (k % observe_steps == 0)
+                    hierarchy_top(tops.data() + (std::size_t)(k / observe_steps * N * N));
+            }
+        } else {
+            hip_call(heom_propagate(h_, step_size_, n, observe_steps, reinterpret_cast<double*>(tops.data())));
+        }
+        step_stats_.add(seconds_since(t0), n);
         step_count_ += n;
         time_ += (double)n * step_size_;
         return tops;
@@ -86,7 +135,9 @@ public:
     void hierarchy(complex_t* buffer) { hip_call(heom_get_state(h_, reinterpret_cast<double*>(buffer))); }
     void hierarchy_norm(real_t* buffer, real_t threshold = -1.0)
     {
+        const auto t0 = std::chrono::steady_clock::now();
         hip_call(heom_hierarchy_norm(h_, threshold, buffer));
+        norm_stats_.add(seconds_since(t0), 1);
     }
     void update_hierarchy_mask(const std::uint8_t* mask) { hip_call(heom_set_mask(h_, mask)); }
     void hierarchy_mmult_left(const complex_t* m)
@@ -109,6 +160,15 @@ public:
         hip_call(heom_timing(h_, &t));
         return t;
     }
+    const time_stats& step_stats() const { return step_stats_; }
+    const time_stats& norm_stats() const { return norm_stats_; }
+    const std::string& stepper() const { return stepper_name_; }
+    std::int64_t device_bytes()
+    {
+        std::int64_t b = 0;
+        hip_call(heom_device_bytes(h_, &b));
+        return b;
+    }
 
 private:
     instance& inst_;
@@ -116,7 +176,51 @@ private:
     real_t step_size_;
     int_t step_count_ = 0;
     real_t time_ = 0.0;
+    std::unique_ptr<explicit_rk_stepper> stepper_;  // null: the fused RK4 step
+    std::string stepper_name_;
+    time_stats step_stats_, norm_stats_;
 };
+
+inline void write_time_statistics_summary(const time_stats& st, const std::string& name, std::ostream& os,
+                                          std::size_t name_width = 25)
+{
+    char line[160];
+    std::snprintf(line, sizeof line, "%11.2f s (count: %6lld, average: %10.2f ms)", st.sum,
+                  (long long)st.count, st.count ? st.sum / (double)st.count * 1e3 : 0.0);
+    os << "time in " << name << ": " << std::string(name_width > name.size() ? name_width - name.size() : 0, ' ')
+       << line << '\n';
+}
+
+// solver_base::write_runtime_summary (dm-heom/include/heom/solver_base.hpp:376-383,
+// format of dm-heom/src/heom/common.cpp:34-45): host time per RK4 step, device time of
+// the stage kernels (4 per fused RK4 step) and the norm kernel
+inline void write_runtime_summary(population_dynamics_solver& s, std::ostream& os)
+{
+    write_time_statistics_summary(s.step_stats(), "solver step (" + s.stepper() + ")", os);
+    const heom_timing_info t = s.timing();
+    time_stats ode;
+    ode.add(t.rk4_seconds, t.rhs_launches);
+    if (t.rk4_seconds > 0.0)
+        write_time_statistics_summary(ode, "heom_ode kernel", os);
+    else
+        os << "heom_ode kernel launches: " << t.rhs_launches << " (timed within the solver steps)\n";
+    write_time_statistics_summary(s.norm_stats(), "hierarchy_norm kernel", os);
+}
+
+// the memory summary of dm-heom/src/app_population_dynamics.cpp:112-127: the hierarchy
+// in the reference layout, the host instance arrays, the device memory of the handle
+inline void write_memory_summary(const instance& inst, population_dynamics_solver& s, std::ostream& os)
+{
+    auto mib = [](double b) {
+        char line[32];
+        std::snprintf(line, sizeof line, "%11.2f MiB", b / 1024.0 / 1024.0);
+        return std::string(line);
+    };
+    const double hierarchy = (double)inst.matrices() * inst.states() * inst.states() * 16.0;
+    os << "local hierarchy buffer size:            " << mib(hierarchy) << '\n';
+    os << "local instance size:                    " << mib((double)inst.allocated_byte()) << '\n';
+    os << "device memory of the handle (HBM):      " << mib((double)s.device_bytes()) << '\n';
+}
 
 // thermal_state_search_solver (dm-heom/include/heom/thermal_state_search_solver.hpp:31-376)
 // over heom_create_tss: the set-up of initialize() (:124-209) on the host, step_forward

@@ -153,6 +153,56 @@ int heom_hierarchy_norm(heom_handle* handle, double threshold, double* norms /* 
  * evaluation (dm-heom/src/heom/ode.cpp:97-117, heom_ode.cl:430-434), as used by the
  * rate-kernel app (dm-heom/src/app_rate_kernel.cpp:91-93). The state is unchanged. */
 int heom_rhs(heom_handle* handle, double* out /* [M*N*N*2] */);
+
+/* ---- per-evaluation interface: the operator between device state buffers -------------
+ *
+ * The reference's ODE wrapper evaluates the operator between device buffers in four
+ * forms (ode::solve, dm-heom/include/heom/ode.hpp:40-43, dm-heom/src/heom/ode.cpp:97-196,
+ * kernel side dm-heom/cl/heom/heom_ode.cl:416-434) and calls its pre-evaluate actions first
+ * (ode.hpp:53-55, ode.cpp:17-21); its steppers (noma/num, not vendored) compose steps from
+ * those calls.  These entry points give a caller the same building blocks on the handle's
+ * state buffers: buffer 0 is the state (heom_set_state, heom_get_state, heom_get_top),
+ * buffers 1-3 are the work buffers of heom_step_rk4 and heom_propagate* (which overwrite
+ * them), buffers 4.. are added with heom_add_buffers.  FP64 handles only (FP32 storage
+ * keeps FP64 shadows of the tier-0/1 ADOs that these buffers do not carry); all calls are
+ * ordered on the handle's stream.  heom_compact keeps buffer 0 and zeroes the others. */
+#define HEOM_MAX_BUFFERS 16
+#define HEOM_NO_BUFFER (-1)
+/* Called by heom_evaluate before the evaluation, in the order added, with heom_evaluate's
+ * time, step_size, in and out; may change the handle's Hamiltonian (heom_set_hamiltonian),
+ * e.g. a time-dependent field.  A non-zero return fails the evaluation with that code's
+ * message. */
+typedef int (*heom_pre_evaluate_fn)(heom_handle* handle, double time, double step_size, int in, int out,
+                                    void* user);
+/* ode::add_pre_evaluate_action / clear_pre_evaluate_actions (ode.hpp:53-55). */
+int heom_add_pre_evaluate(heom_handle* handle, heom_pre_evaluate_fn fn, void* user);
+int heom_clear_pre_evaluate(heom_handle* handle);
+/* f = L(buffer in), then
+ *   out >= 0:  buffer out = (out_base >= 0 ? buffer out_base : 0) + out_coeff * f
+ *   acc >= 0:  buffer acc = (acc_init ? buffer in : buffer acc) + acc_coeff * f
+ * (out != acc; either may be the same buffer as in or out_base).  The reference's forms:
+ *   ode::solve(t, h, in, out)                          -> (t, h, in, out, -1, 1, -1, 0, 0)
+ *   ode::solve(t, h, in, out, acc, c, init)            -> (t, h, in, out, -1, 1, acc, c, init)
+ *   ode::solve(t, h, in, out, oc, acc, c, init) with the sub-diagonal stage input
+ *                                                      -> (t, h, in, out, base, oc, acc, c, init)
+ * (the form ode::solve(t, h, in, out, c) scales only the flow tracking, which this build
+ * does not carry, and is the pure evaluation here).  Asynchronous. */
+int heom_evaluate(heom_handle* handle, double time, double step_size, int in, int out, int out_base,
+                  double out_coeff, int acc, double acc_coeff, int acc_init);
+/* buffer dst = sum_k coeff[k] * buffer src[k], 1 <= n <= 8; dst may be a source.  No
+ * reference counterpart (noma/num's weighted adds); the building block of general
+ * explicit Runge-Kutta tableaus.  Asynchronous. */
+int heom_lincomb(heom_handle* handle, int dst, int n, const int* src, const double* coeff);
+/* exchanges two buffers (their device addresses); buffer 0 stays "the state". */
+int heom_swap_buffers(heom_handle* handle, int a, int b);
+/* adds n zeroed state buffers (at most HEOM_MAX_BUFFERS in all); *count = buffers now
+ * (count may be NULL). */
+int heom_add_buffers(heom_handle* handle, int n, int* count);
+int heom_buffer_count(heom_handle* handle, int* count);
+/* a buffer from / into the reference layout, like heom_set_state / heom_get_state. */
+int heom_set_buffer(heom_handle* handle, int b, const double* state /* [M*N*N*2] */);
+int heom_get_buffer(heom_handle* handle, int b, double* state /* [M*N*N*2] */);
+
 /* nsteps classical RK4 steps of size h (seconds): population_dynamics_solver::
  * step_forward (population_dynamics_solver.hpp:233-265) with the stepper's heom_ode and
  * rk_weighted_add launches fused into one kernel per stage.  Asynchronous. */

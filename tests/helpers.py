@@ -55,3 +55,24 @@ def max_abs_rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def make_instance(N, B, K, D, coupling, seed=0, complex_h=False, omega=None, temperature=300.0):
+    """A seeded instance of N sites (H about 12,000-12,600 cm^-1, couplings up to 100 cm^-1),
+    B Drude-Lorentz baths of K modes each (lambda 20-60 cm^-1, 1/gamma 30-120 fs), depth D;
+    coupling[b] lists the states bath b couples to."""
+    rng = np.random.default_rng(seed)
+    H = np.diag(rng.uniform(12000, 12600, N)).astype(np.complex128)
+    for i in range(N):
+        for j in range(i + 1, N):
+            v = rng.uniform(-100, 100) + (1j * rng.uniform(-40, 40) if complex_h else 0.0)
+            H[i, j] = v
+            H[j, i] = np.conj(v)
+    H *= O.INVCM_TO_JOULE
+    lam = rng.uniform(20, 60, B) * O.INVCM_TO_JOULE
+    invnu = rng.uniform(30, 120, B) * 1e-15
+    om = (np.zeros(B) if omega is None else np.asarray(omega)) * (O.INVCM_TO_JOULE / O.HBAR)
+    g = O.build_graph(B * K, D)
+    return O.Instance(states=N, matsubaras=K, baths=B, coupling=np.asarray(coupling, np.int32),
+                      hamiltonian=H, graph=g, lambda_=lam, invnu=invnu, omega=om,
+                      temperature=temperature)

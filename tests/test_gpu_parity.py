@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import as_complex, instance, load_config, max_abs_rel, random_state, rel_err
+from helpers import as_complex, instance, load_config, make_instance, max_abs_rel, random_state, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -24,24 +24,6 @@ def stage_kernel(request, monkeypatch):
     workgroups (the default for small hierarchies; where N allows)."""
     monkeypatch.setenv("HEOM_STAGE_KERNEL", request.param)
     return request.param
-
-
-def make_instance(N, B, K, D, coupling, seed=0, complex_h=False, omega=None, temperature=300.0):
-    rng = np.random.default_rng(seed)
-    H = np.diag(rng.uniform(12000, 12600, N)).astype(np.complex128)
-    for i in range(N):
-        for j in range(i + 1, N):
-            v = rng.uniform(-100, 100) + (1j * rng.uniform(-40, 40) if complex_h else 0.0)
-            H[i, j] = v
-            H[j, i] = np.conj(v)
-    H *= O.INVCM_TO_JOULE
-    lam = rng.uniform(20, 60, B) * O.INVCM_TO_JOULE
-    invnu = rng.uniform(30, 120, B) * 1e-15
-    om = (np.zeros(B) if omega is None else np.asarray(omega)) * (O.INVCM_TO_JOULE / O.HBAR)
-    g = O.build_graph(B * K, D)
-    return O.Instance(states=N, matsubaras=K, baths=B, coupling=np.asarray(coupling, np.int32),
-                      hamiltonian=H, graph=g, lambda_=lam, invnu=invnu, omega=om,
-                      temperature=temperature)
 
 
 def expected_kernel(inst, requested, complex_bytes=16):

@@ -108,6 +108,24 @@ def test_config_errors(tmp_path, old, new, msg):
         O.parse_config(str(p))
 
 
+def test_stepper_types(tmp_path):
+    """solver.stepper_type takes every tableau of heom_amd.steppers (the reference's list
+    comes from the un-vendored noma/num; its examples use rk_rk4) and names the accepted
+    ones otherwise."""
+    from heom_amd import steppers
+
+    text = open(config_path("excitonic7_d4")).read()
+    assert "stepper_type=rk_rk4" in text
+    for name in steppers.TABLEAUS:
+        p = tmp_path / f"{name}.cfg"
+        p.write_text(text.replace("stepper_type=rk_rk4", f"stepper_type={name}", 1))
+        heom_amd.HostInstance(str(p)).close()
+    p = tmp_path / "bogus.cfg"
+    p.write_text(text.replace("stepper_type=rk_rk4", "stepper_type=rk_bogus", 1))
+    with pytest.raises(heom_amd.ConfigError, match="rk_bogus' is not supported .one of rk_euler"):
+        heom_amd.HostInstance(str(p))
+
+
 def test_non_hermitian_hamiltonian_rejected(tmp_path):
     text = open(config_path("excitonic7_d4")).read()
     text = re.sub(r"hamiltonian=\{\{([0-9.]+), ([0-9.-]+),", r"hamiltonian={{\1, 1.5,", text)
