@@ -179,25 +179,56 @@ __global__ __launch_bounds__(64) void observe_tops(const double2* __restrict__ t
 
 // per-ADO squared Frobenius norm in uid order; threshold masking as in
 // dm-heom/cl/heom/hierarchy_norm.cl:41-44 (mask in uid order)
-template <typename C>
-__global__ void hierarchy_norm_kernel(const C* __restrict__ y, const int32_t* __restrict__ q2u, int64_t mp,
-                                      int N, bool paired, double threshold, double* __restrict__ norms,
-                                      uint8_t* __restrict__ mask)
+// hierarchy_norm (dm-heom/cl/heom/hierarchy_norm.cl:15-45): one workgroup per 64-ADO tile,
+// lane = ADO (so every load is a coalesced plane segment), the tile's physical planes
+// dealt round-robin to kNormWaves wavefronts, whose partial sums wavefront 0 adds in a
+// fixed order.  A 16-byte slot of the paired complex64 layout holds two elements; the
+// squares of all its components are summed.  Splitting the planes keeps small hierarchies
+// of large matrices (20,475 ADOs of 24 x 24 at 576 planes) from running one long serial
+// loop per thread on a few waves per CU.
+constexpr int kNormWaves = 8;
+
+template <typename V>
+__global__ __launch_bounds__(64 * kNormWaves) void hierarchy_norm_kernel(const V* __restrict__ planes,
+                                                                        int nplanes, const int32_t* __restrict__ q2u,
+                                                                        int64_t mp, double threshold,
+                                                                        double* __restrict__ norms,
+                                                                        uint8_t* __restrict__ mask)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= mp)
+    __shared__ double part[kNormWaves][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 64 + lane;
+    double acc = 0.0;
+    if (q < mp) {
+        for (int d = w; d < nplanes; d += kNormWaves) {
+            const V v = planes[(int64_t)d * mp + q];
+            if constexpr (sizeof(V) == 16 && std::is_same<V, double2>::value) {
+                acc = fma(v.x, v.x, acc);
+                acc = fma(v.y, v.y, acc);
+            } else if constexpr (sizeof(V) == 16) {  // float4: two complex64 elements
+                acc = fma((double)v.x, (double)v.x, acc);
+                acc = fma((double)v.y, (double)v.y, acc);
+                acc = fma((double)v.z, (double)v.z, acc);
+                acc = fma((double)v.w, (double)v.w, acc);
+            } else {  // float2
+                acc = fma((double)v.x, (double)v.x, acc);
+                acc = fma((double)v.y, (double)v.y, acc);
+            }
+        }
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w != 0 || q >= mp)
         return;
     const int32_t u = q2u[q];
     if (u < 0)
         return;
-    double acc = 0.0;
-    for (int e = 0; e < N * N; ++e) {
-        const double2 v = to_d2(y[plane_index<C>(paired, N, e, q, mp)]);
-        acc = fma(v.x, v.x, acc);
-        acc = fma(v.y, v.y, acc);
-    }
-    norms[u] = acc;
-    if (acc - threshold <= 0.0 && mask[u] != 2)
+    double sum = part[0][lane];
+#pragma unroll
+    for (int k = 1; k < kNormWaves; ++k)
+        sum += part[k][lane];
+    norms[u] = sum;
+    if (sum - threshold <= 0.0 && mask[u] != 2)
         mask[u] = 0;
 }
 
@@ -1768,13 +1799,16 @@ int heom_hierarchy_norm(heom_handle* h, double threshold, double* norms)
             raise(HEOM_E_INVALID, "norms is NULL");
         if (h->active < h->M)  // ADOs removed by heom_compact have norm 0
             HIPC(hipMemsetAsync(h->norms.p, 0, h->norms.bytes(), h->stream));
-        if (h->fp32())
-            hipLaunchKernelGGL(hierarchy_norm_kernel<float2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0, h->stream,
-                               h->y.as<float2>(), h->q2u.p, h->Mp, h->N, h->paired, threshold, h->norms.p, h->mask.p);
+        const dim3 grid((unsigned)(h->Mp / kTile)), block(64 * kNormWaves);
+        if (h->fp32() && h->paired)
+            hipLaunchKernelGGL(hierarchy_norm_kernel<float4>, grid, block, 0, h->stream, h->y.as<float4>(),
+                               pair_planes(h->N), h->q2u.p, h->Mp, threshold, h->norms.p, h->mask.p);
+        else if (h->fp32())
+            hipLaunchKernelGGL(hierarchy_norm_kernel<float2>, grid, block, 0, h->stream, h->y.as<float2>(), h->nel,
+                               h->q2u.p, h->Mp, threshold, h->norms.p, h->mask.p);
         else
-            hipLaunchKernelGGL(hierarchy_norm_kernel<double2>, dim3(blocks_for(h->Mp, 256)), dim3(256), 0,
-                               h->stream, h->y.as<double2>(), h->q2u.p, h->Mp, h->N, false, threshold, h->norms.p,
-                               h->mask.p);
+            hipLaunchKernelGGL(hierarchy_norm_kernel<double2>, grid, block, 0, h->stream, h->y.as<double2>(), h->nel,
+                               h->q2u.p, h->Mp, threshold, h->norms.p, h->mask.p);
         HIPC(hipGetLastError());
         if (threshold >= 0.0) {
             refold_tables(h);

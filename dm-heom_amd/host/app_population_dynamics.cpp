@@ -33,16 +33,17 @@ int main(int argc, char** argv)
                   << graph.width() << ", depth " << graph.depth() << std::endl;
 
         std::vector<std::unique_ptr<heom::file_observer>> observers;
-        bool norms = false;
+        bool norms = false, whole = false;
         for (auto& o : cfg.observations()) {
             observers.emplace_back(new heom::file_observer(o.type, o.filename, inst));
             norms = norms || observers.back()->needs_norms();
+            whole = whole || observers.back()->needs_hierarchy();
         }
 
         const heom::int_t N = inst.states();
         const heom::int_t iterations = cfg.solver_steps() / cfg.program_observe_steps();
         std::vector<heom::complex_t> top((std::size_t)(N * N));
-        if (!norms) {
+        if (!norms && !whole) {
             // every observation is a function of the top ADO: record it on the device
             const auto tops = solver.propagate_observe(cfg.solver_steps(), cfg.program_observe_steps());
             for (heom::int_t r = 0; r <= iterations; ++r) {
@@ -51,15 +52,21 @@ int main(int argc, char** argv)
                     ob->observe(time, &tops[(std::size_t)(r * N * N)]);
             }
         } else {
+            // observations that need the norms or the whole hierarchy read them back at
+            // every observation
             std::vector<double> nv((std::size_t)graph.nodes());
+            std::vector<heom::complex_t> hv(whole ? (std::size_t)(graph.nodes() * N * N) : 0);
             for (heom::int_t r = 0; r <= iterations; ++r) {
                 if (r > 0)
                     solver.step_forward(cfg.program_observe_steps());
                 solver.hierarchy_top(top.data());
-                solver.hierarchy_norm(nv.data());
+                if (norms)
+                    solver.hierarchy_norm(nv.data());
+                if (whole)
+                    solver.hierarchy(hv.data());
                 const double time = (double)(r * cfg.program_observe_steps()) * cfg.solver_step_size();
                 for (auto& ob : observers)
-                    ob->observe(time, top.data(), &nv);
+                    ob->observe(time, top.data(), &nv, &hv);
                 if (!cl.no_progress)
                     std::cout << "\rCalculation population dynamics: " << (100 * r / iterations) << " %" << std::flush;
             }

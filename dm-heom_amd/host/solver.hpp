@@ -312,15 +312,18 @@ public:
         if (out_.fail())
             throw std::runtime_error("Error: could not open output file: " + filename);
         if (type != "matrix_diagonal" && type != "matrix_complete" && type != "matrix_trace_id" &&
-            type != "hierarchy_norm")
+            type != "hierarchy_norm" && type != "hierarchy_complete")
             throw std::runtime_error("observation type '" + type + "' is not supported by this application");
         write_header();
     }
 
     bool needs_norms() const { return type_ == "hierarchy_norm"; }
+    bool needs_hierarchy() const { return type_ == "hierarchy_complete"; }
 
-    // write one row from the top ADO (and the norms for hierarchy_norm)
-    void observe(double time, const complex_t* top, const std::vector<double>* norms = nullptr)
+    // write one row from the top ADO (and the norms for hierarchy_norm, the whole
+    // hierarchy in uid order for hierarchy_complete)
+    void observe(double time, const complex_t* top, const std::vector<double>* norms = nullptr,
+                 const std::vector<complex_t>* hierarchy = nullptr)
     {
         const int_t N = inst_.states();
         out_ << fmt_real(time);
@@ -335,6 +338,11 @@ public:
             for (int_t i = 0; i < N; ++i)
                 tr += top[i * N + i];
             out_ << kDelimiter << fmt_real(tr.real()) << kDelimiter << fmt_real(tr.imag());
+        } else if (type_ == "hierarchy_complete") {  // hierarchy_complete_observer.hpp:26-32
+            if (!hierarchy)
+                throw std::runtime_error("hierarchy_complete observation without the hierarchy");
+            for (const complex_t& v : *hierarchy)
+                out_ << kDelimiter << fmt_real(v.real()) << kDelimiter << fmt_real(v.imag());
         } else if (norms) {  // hierarchy_norm_observer.hpp
             for (double v : *norms)
                 out_ << kDelimiter << fmt_real(v);
@@ -357,6 +365,12 @@ private:
                          << j << "_imag";
         } else if (type_ == "matrix_trace_id") {
             out_ << kDelimiter << "trace_real" << kDelimiter << "trace_imag";
+        } else if (type_ == "hierarchy_complete") {  // hierarchy_complete_observer.hpp:58-70
+            for (int_t h = 0; h < inst_.matrices(); ++h)
+                for (int_t i = 0; i < N; ++i)
+                    for (int_t j = 0; j < N; ++j)
+                        out_ << kDelimiter << "mat_" << h << "_elem_" << i << "_" << j << "_real" << kDelimiter
+                             << "mat_" << h << "_elem_" << i << "_" << j << "_imag";
         } else {
             const auto& g = inst_.graph();
             for (int_t u = 0; u < g.nodes(); ++u) {

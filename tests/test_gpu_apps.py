@@ -72,6 +72,37 @@ def test_population_dynamics_app_fmo(oracle_mod, tmp_path):
     assert np.max(np.abs(tr[:, 1] - 1.0)) < 1e-12
 
 
+def test_hierarchy_complete_observation(oracle_mod, tmp_path):
+    """hierarchy_complete (dm-heom/include/heom/hierarchy_complete_observer.hpp:42-82): every
+    ADO in uid order at every observation, FMO for 20 steps observed every 10, against the
+    oracle's state (1e-10 relative to the largest value), and the reference's column names."""
+    cfg_path = write_cfg(os.path.join(FMO_DIR, "fmo_population_dynamics_77K.cfg"), tmp_path / "pd.cfg",
+                         steps="20", observe_steps="10",
+                         observations="{(hierarchy_complete, pd_all.dat), (matrix_diagonal, pd_diag.dat)}")
+    run_app("heom_population_dynamics", str(cfg_path), tmp_path)
+    header, got = read_table(tmp_path / "pd_all.dat")
+    cfg = O.parse_config(str(cfg_path))
+    inst = O.instance_from_config(cfg)
+    N, M = inst.states, inst.matrices
+    assert header[:4] == ["time", "mat_0_elem_0_0_real", "mat_0_elem_0_0_imag", "mat_0_elem_0_1_real"]
+    assert header[-1] == f"mat_{M - 1}_elem_{N - 1}_{N - 1}_imag" and len(header) == 1 + M * N * N * 2
+    y = inst.state_with_top(cfg.rho_init)
+    rows = [y.copy()]
+    work = np.empty(5 * inst.state_size)
+    for n in range(1, cfg.steps + 1):
+        inst.rk4_step(y, cfg.step_size, work)
+        if n % cfg.observe_steps == 0:
+            rows.append(y.copy())
+    want = np.array(rows)
+    assert got.shape == (3, 1 + M * N * N * 2)
+    assert np.allclose(got[:, 0], np.arange(3) * 10 * cfg.step_size, rtol=1e-15, atol=0)
+    err = np.max(np.abs(got[:, 1:] - want)) / np.max(np.abs(want))
+    print(f"hierarchy_complete: {M} ADOs x 3 observations, max rel deviation {err:.3e}", flush=True)
+    assert err < 1e-10
+    _, diag = read_table(tmp_path / "pd_diag.dat")
+    assert diag.shape == (3, 1 + N)
+
+
 @pytest.mark.parametrize("stepper", ["rk_heun", "rk_kutta3", "rk_rk4_38"])
 def test_population_dynamics_app_steppers(oracle_mod, tmp_path, stepper):
     """solver.stepper_type other than rk_rk4: the app steps through the per-evaluation
