@@ -46,7 +46,8 @@ def parse_args():
                    help="skip the FP32-storage run of the same workload and the batched ensemble")
     p.add_argument("--extras", default="precision,ensemble,adjoint,filtering,field,tss",
                    help="comma-separated extra sections to run: precision (FP32 storage, 24 sites), ensemble, "
-                        "adjoint, filtering (compaction), field (time-dependent H), tss (thermal state search)")
+                        "adjoint, filtering (compaction), field (time-dependent H), tss (thermal state search), "
+                        "norms (hierarchy norms; not in the default set)")
     p.add_argument("--ensemble-members", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
@@ -359,6 +360,31 @@ def field_run(host, steps: int = 40) -> dict:
             "ms_per_step_field": t_field * 1e3, "ms_per_step_constant_h": t_plain * 1e3}
 
 
+def norms_run(calls: int = 12) -> dict:
+    """heom_hierarchy_norm (the norms of every ADO, read back to the host) on the 8-site
+    depth-8 and the 24-site depth-4 hierarchies in FP64 and FP32 storage, after two RK4
+    steps: milliseconds per call (kernel, 8 bytes per ADO to the host and the sync) and
+    the state bytes each call reads."""
+    import heom_amd
+
+    out = {}
+    for cfg in ("excitonic8_d8", "excitonic24_d4"):
+        host = heom_amd.HostInstance(os.path.join(heom_amd.CONFIG_DIR, f"{cfg}.cfg"))
+        for prec, name in ((heom_amd.HEOM_PRECISION_FP64, "fp64"), (heom_amd.HEOM_PRECISION_FP32, "fp32")):
+            h = host.create_handle(precision=prec)
+            h.step_rk4(host.info.step_size, 2)
+            h.norms()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                h.norms()
+            dt = (time.perf_counter() - t0) / calls
+            out[f"{cfg}/{name}"] = {"ms_per_call": dt * 1e3, "ados": host.matrices,
+                                    "state_bytes": host.matrices * host.states ** 2 * (8 if name == "fp32" else 16)}
+            h.close()
+        host.close()
+    return out
+
+
 def tss_run(host, iterations: int = 20) -> dict:
     """Thermal state search iterations (heom_tss_step: one f_TSS evaluation by the stage
     kernel plus the per-ADO eigenbasis transform) on the same hierarchy, timed on the
@@ -557,6 +583,8 @@ def main():
         out["field"] = field_run(host)
     if "tss" in extras:
         out["tss"] = tss_run(host)
+    if "norms" in extras:
+        out["norms"] = norms_run()
     if "ensemble" in extras:
         out["ensemble"] = ensemble_run(args.ensemble_members)
         out["ensemble"]["fp32_ensemble_vs_plain_fp64"] = out["ensemble"]["fp32"]["value"] / value
