@@ -79,27 +79,34 @@ __device__ __forceinline__ void plane_elements(int d, int& r0, int& c0, int& r1,
 template <int N, int R, bool REALH>
 __device__ __forceinline__ void row_terms(const char* vb, const float2* hcol, const float2* hrow, float2& u, float2& v)
 {
-    f4 A[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k)
-        A[k] = *reinterpret_cast<const f4*>(vb + pair_slot(N, R, k).x * (int)(sizeof(f4) * kTile));
+    // the N reads in two batches of about N/2, each summed before the next is issued:
+    // half the registers of one batch (eight-term planes then fit at N = 8)
     u = v = make_float2(0.f, 0.f);
+    constexpr int HALF = (N + 1) / 2;
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const int h = pair_slot(N, R, k).y;  // half holding sigma_{R k}; sigma_{k R} is the other
-        const int ht = k == R ? h : 1 - h;   // (the same one on the diagonal)
-        const float2 s_rk = h ? make_float2(A[k].z, A[k].w) : make_float2(A[k].x, A[k].y);
-        const float2 s_kr = ht ? make_float2(A[k].z, A[k].w) : make_float2(A[k].x, A[k].y);
-        const float2 hkp = hcol[k * N], hpk = hrow[k];
-        if (REALH) {
-            u.x = fmaf(s_rk.x, hkp.x, u.x);
-            u.y = fmaf(s_rk.y, hkp.x, u.y);
-            v.x = fmaf(-hpk.x, s_kr.x, v.x);
-            v.y = fmaf(-hpk.x, s_kr.y, v.y);
-        } else {
-            u = cfma(s_rk, hkp, u);
-            v = cfma(make_float2(-hpk.x, -hpk.y), s_kr, v);
+    for (int k0 = 0; k0 < N; k0 += HALF) {
+        f4 A[N];
+#pragma unroll
+        for (int k = k0; k < k0 + HALF && k < N; ++k)
+            A[k] = *reinterpret_cast<const f4*>(vb + pair_slot(N, R, k).x * (int)(sizeof(f4) * kTile));
+#pragma unroll
+        for (int k = k0; k < k0 + HALF && k < N; ++k) {
+            const int h = pair_slot(N, R, k).y;  // half holding sigma_{R k}; sigma_{k R} is the other
+            const int ht = k == R ? h : 1 - h;   // (the same one on the diagonal)
+            const float2 s_rk = h ? make_float2(A[k].z, A[k].w) : make_float2(A[k].x, A[k].y);
+            const float2 s_kr = ht ? make_float2(A[k].z, A[k].w) : make_float2(A[k].x, A[k].y);
+            const float2 hkp = hcol[k * N], hpk = hrow[k];
+            if (REALH) {
+                u.x = fmaf(s_rk.x, hkp.x, u.x);
+                u.y = fmaf(s_rk.y, hkp.x, u.y);
+                v.x = fmaf(-hpk.x, s_kr.x, v.x);
+                v.y = fmaf(-hpk.x, s_kr.y, v.y);
+            } else {
+                u = cfma(s_rk, hkp, u);
+                v = cfma(make_float2(-hpk.x, -hpk.y), s_kr, v);
+            }
         }
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
