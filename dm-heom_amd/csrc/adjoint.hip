@@ -12,10 +12,15 @@
 // gradient with respect to any coefficient kappa of L,
 //   dt/6 <A4, dL/dkappa Y1> + dt/3 <A3, dL/dkappa Y2> + dt/3 <A2, dL/dkappa Y3> + dt/6 <A1, dL/dkappa Y4>,
 // with <a, b> = Re sum conj(a) b (the RHS is complex-linear, so its real transpose is
-// L^dag).  The forward sweep keeps the state every checkpoint_steps steps; the backward
-// sweep recomputes each segment's states from its checkpoint, and each step's forward
-// stage vectors from its state, next to the adjoint stages; one contraction launch per
-// step covers all four stage pairs.
+// L^dag).  When the state and the three forward stage vectors of every step fit in half
+// the free device memory (12 GB for 1000 steps of 3003 ADOs of 8 x 8), the forward sweep
+// keeps them all and the backward sweep recomputes nothing: per step its three adjoint
+// stages, one contraction launch covering all four stage pairs, and the fourth stage.
+// Otherwise (or with HEOM_ADJOINT_RECOMPUTE=1 in the environment) the forward sweep keeps
+// the state every checkpoint_steps steps, and the backward sweep recomputes each
+// segment's states from its checkpoint and each step's forward stage vectors from its
+// state.  Both modes run the same kernels on the same inputs: their gradients agree to
+// the last bit.
 //
 // The contraction kernel accumulates, per tile of 64 ADOs and in FP64, the sums the
 // gradients of all coefficient arrays of heom_desc follow from:
@@ -30,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -265,6 +271,34 @@ __global__ void add_tops(double2* __restrict__ lam, const double2* __restrict__ 
     *p = make_double2(p->x + src[t].x, p->y + src[t].y);
 }
 
+// populations and Tr(O sigma_0) of every instance straight from state planes (the top
+// ADO's element e is plane e at slot qs[k]): observe_tops' sums in its order, in one
+// launch instead of a gather and a reduction
+__global__ __launch_bounds__(64) void observe_planes(const double2* __restrict__ y, const int64_t* __restrict__ qs,
+                                                     int64_t mp, const double2* __restrict__ ops,
+                                                     double* __restrict__ pops, double2* __restrict__ traces, int N)
+{
+    const int k = blockIdx.x, lane = threadIdx.x, nel = N * N;
+    const int64_t qk = qs[k];
+    double re = 0.0, im = 0.0;
+    for (int e = lane; e < nel; e += 64) {
+        const int i = e / N, j = e % N;
+        const double2 s = y[(int64_t)e * mp + qk];
+        if (i == j)
+            pops[(int64_t)k * N + i] = s.x;
+        // Tr(D sigma) = sum_ij D_ji sigma_ij
+        const double2 d = ops[(int64_t)k * nel + j * N + i];
+        re += fma(d.x, s.x, -d.y * s.y);
+        im += fma(d.x, s.y, d.y * s.x);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
+    }
+    if (lane == 0)
+        traces[k] = make_double2(re, im);
+}
+
 template <int N, int NT>
 void launch_contract_t(const ContractArgs& a, const Term* terms, int ntiles, hipStream_t st)
 {
@@ -377,15 +411,62 @@ size_t plane_bytes(const heom_handle* h) { return h->y.bytes(); }
 
 unsigned char* slot(DevBuf<unsigned char>& b, size_t bytes, int64_t k) { return b.p + (size_t)k * bytes; }
 
-// dst planes = one RK4 step of src (FP64, explicit buffers); f2 / f3 / f4 receive Y2, Y3, Y4
+// dst planes = one RK4 step of src (FP64, explicit buffers); y2 / y3 / y4 receive the
+// stage vectors Y2, Y3, Y4 (no fourth stage for dst = null)
+void rk4_step_stages(heom_handle* h, const unsigned char* src, unsigned char* y2, unsigned char* y3,
+                     unsigned char* y4, unsigned char* dst, double dt)
+{
+    launch_stage_buffers(h, 1, src, src, h->pbuf.p, y2, 0.5 * dt, false, false);
+    launch_stage_buffers(h, 2, y2, src, h->pbuf.p, y3, 0.5 * dt, false, false);
+    launch_stage_buffers(h, 3, y3, src, h->pbuf.p, y4, dt, false, false);
+    if (dst)
+        launch_stage_buffers(h, 4, y4, src, h->pbuf.p, dst, dt / 6.0, false, false);
+}
+
+// the same with the stage vectors in f2 / f3 / f4
 void rk4_step_into(heom_handle* h, const unsigned char* src, unsigned char* dst, double dt)
 {
     AdjointState& A = *h->adj;
-    launch_stage_buffers(h, 1, src, src, h->pbuf.p, A.f2.p, 0.5 * dt, false, false);
-    launch_stage_buffers(h, 2, A.f2.p, src, h->pbuf.p, A.f3.p, 0.5 * dt, false, false);
-    launch_stage_buffers(h, 3, A.f3.p, src, h->pbuf.p, A.f4.p, dt, false, false);
-    if (dst)
-        launch_stage_buffers(h, 4, A.f4.p, src, h->pbuf.p, dst, dt / 6.0, false, false);
+    rk4_step_stages(h, src, A.f2.p, A.f3.p, A.f4.p, dst, dt);
+}
+
+// one backward RK4 step: lambda = dJ/dy_{n+1} becomes dJ/dy_n, with the step's gradient
+// contributions added to the tile sums; yn, y2, y3, y4 = the step's forward stage vectors
+void adjoint_step(heom_handle* h, const unsigned char* yn, const unsigned char* y2, const unsigned char* y3,
+                  const unsigned char* y4, double dt)
+{
+    AdjointState& A = *h->adj;
+    // adjoint stages A2 (a), A3 (b), A4 (c) from A1 = lambda, then all four stage pairs in
+    // one contraction, then dJ/dy_n into lambda
+    launch_stage_buffers(h, 1, A.lam.p, A.lam.p, h->pbuf.p, A.a.p, 0.5 * dt, false, true);
+    launch_stage_buffers(h, 2, A.a.p, A.lam.p, h->pbuf.p, A.b.p, 0.5 * dt, false, true);
+    launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.c.p, dt, false, true);
+    const void* mus[kPairs] = {A.lam.p, A.a.p, A.b.p, A.c.p};
+    const void* ys[kPairs] = {y4, y3, y2, yn};
+    launch_contract(h, mus, ys, dt);
+    launch_stage_buffers(h, 4, A.c.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
+}
+
+// observation row `row` of the forward sweep from the state planes y
+void observe_row(heom_handle* h, const unsigned char* y, const double2* ops, double* pops, double2* traces,
+                 int64_t row)
+{
+    hipLaunchKernelGGL(observe_planes, dim3(h->count), dim3(64), 0, h->stream, reinterpret_cast<const double2*>(y),
+                       h->q_top.p, h->Mp, ops, pops + row * h->count * h->N, traces + row * h->count, h->N);
+    HIPC(hipGetLastError());
+}
+
+// whole-trajectory mode: the [last][4] stage planes fit in half the free device memory
+// (the buffer already held counts as free) and HEOM_ADJOINT_RECOMPUTE is not 1
+bool keep_trajectory(heom_handle* h, int64_t last, size_t pb)
+{
+    const char* env = std::getenv("HEOM_ADJOINT_RECOMPUTE");
+    if (last < 1 || (env && std::strcmp(env, "1") == 0))
+        return false;
+    size_t free = 0, total = 0;
+    HIPC(hipMemGetInfo(&free, &total));
+    const double need = 4.0 * (double)pb * (double)last;
+    return need <= 0.5 * ((double)free + (double)h->adj->traj.bytes());
 }
 
 // lambda_top += the adjoint source of observation row `row` (rows with all weights zero
@@ -450,7 +531,19 @@ int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t obse
         const int64_t last = (rows - 1) * observe_steps;
         const int64_t ncp = last / checkpoint_steps + 1;
         const size_t pb = plane_bytes(h);
-        A.cps.alloc(pb * (size_t)ncp);
+        const bool keep = keep_trajectory(h, last, pb);
+        if (keep) {
+            A.cps.free();
+            const size_t need = 4 * pb * (size_t)last;
+            if (A.traj.bytes() != need) {
+                A.traj.alloc(need);
+                // padding slots of the planes must read as zero; the stage kernels keep them so
+                HIPC(hipMemsetAsync(A.traj.p, 0, need, h->stream));
+            }
+        } else {
+            A.traj.free();
+            A.cps.alloc(pb * (size_t)ncp);
+        }
         const int n = h->count * h->nel;
         A.op.assign(ops, ops + 2 * (size_t)n);
         DevBuf<double2> opsd, trec;
@@ -465,9 +558,22 @@ int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t obse
         HIPC(hipEventCreate(&e1));
         try {
             HIPC(hipEventRecord(e0, h->stream));
-            observe_launch(h, opsd.p, prec.p, trec.p);
-            HIPC(hipMemcpyAsync(A.cps.p, h->y.p, pb, hipMemcpyDeviceToDevice, h->stream));
-            int64_t done = 0;
+            observe_row(h, h->y.p, opsd.p, prec.p, trec.p, 0);
+            if (keep) {
+                // step n reads slot 4n and writes its stage vectors to 4n + 1 .. 4n + 3 and
+                // y_{n+1} to slot 4n + 4 (the last one to the state)
+                HIPC(hipMemcpyAsync(A.traj.p, h->y.p, pb, hipMemcpyDeviceToDevice, h->stream));
+                for (int64_t n = 0; n < last; ++n) {
+                    unsigned char* nxt = n + 1 < last ? slot(A.traj, pb, 4 * (n + 1)) : h->y.p;
+                    rk4_step_stages(h, slot(A.traj, pb, 4 * n), slot(A.traj, pb, 4 * n + 1),
+                                    slot(A.traj, pb, 4 * n + 2), slot(A.traj, pb, 4 * n + 3), nxt, dt);
+                    if ((n + 1) % observe_steps == 0)
+                        observe_row(h, nxt, opsd.p, prec.p, trec.p, (n + 1) / observe_steps);
+                }
+            } else {
+                HIPC(hipMemcpyAsync(A.cps.p, h->y.p, pb, hipMemcpyDeviceToDevice, h->stream));
+            }
+            int64_t done = keep ? last : 0;
             while (done < last) {
                 // advance to the next observation or checkpoint, whichever comes first
                 const int64_t to_obs = observe_steps - done % observe_steps;
@@ -476,8 +582,7 @@ int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t obse
                 rk4_steps(h, dt, n_adv);
                 done += n_adv;
                 if (done % observe_steps == 0)
-                    observe_launch(h, opsd.p, prec.p + (done / observe_steps) * h->count * h->N,
-                                   trec.p + (done / observe_steps) * h->count);
+                    observe_row(h, h->y.p, opsd.p, prec.p, trec.p, done / observe_steps);
                 if (done % checkpoint_steps == 0)
                     HIPC(hipMemcpyAsync(slot(A.cps, pb, done / checkpoint_steps), h->y.p, pb,
                                         hipMemcpyDeviceToDevice, h->stream));
@@ -504,6 +609,7 @@ int heom_adjoint_forward(heom_handle* h, double dt, int64_t nsteps, int64_t obse
         A.rows = rows;
         A.last = last;
         A.ncp = ncp;
+        A.keep = keep;
         A.recorded = true;
     });
 }
@@ -541,8 +647,15 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
                     }
             }
 
-        A.seg.alloc(pb * (size_t)std::min<int64_t>(ck, std::max<int64_t>(A.last, 1)));
-        for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.c})
+        if (A.keep) {
+            for (auto* b : {&A.seg, &A.f2, &A.f3, &A.f4})
+                b->free();
+        } else {
+            A.seg.alloc(pb * (size_t)std::min<int64_t>(ck, std::max<int64_t>(A.last, 1)));
+            for (auto* b : {&A.f2, &A.f3, &A.f4})
+                b->alloc(pb);
+        }
+        for (auto* b : {&A.lam, &A.a, &A.b, &A.c})
             b->alloc(pb);
         A.part.alloc((size_t)h->ntiles * kPairs * slots);
         A.gs.alloc((size_t)kPairs * h->Mp);
@@ -550,7 +663,8 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
         HIPC(hipMemcpyAsync(A.src.p, src.data(), src.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
         // padding slots of the state planes must read as zero
         for (auto* b : {&A.f2, &A.f3, &A.f4, &A.lam, &A.a, &A.b, &A.c, &A.seg})
-            HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
+            if (b->p)
+                HIPC(hipMemsetAsync(b->p, 0, b->bytes(), h->stream));
         HIPC(hipMemsetAsync(A.part.p, 0, A.part.bytes(), h->stream));
         HIPC(hipMemsetAsync(A.gs.p, 0, A.gs.bytes(), h->stream));
 
@@ -560,7 +674,14 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
         try {
             HIPC(hipEventRecord(e0, h->stream));
             add_source(h, nonzero, A.rows - 1);
-            for (int64_t c = A.last > 0 ? (A.last - 1) / ck : -1; c >= 0; --c) {
+            if (A.keep)
+                for (int64_t n = A.last - 1; n >= 0; --n) {
+                    adjoint_step(h, slot(A.traj, pb, 4 * n), slot(A.traj, pb, 4 * n + 1), slot(A.traj, pb, 4 * n + 2),
+                                 slot(A.traj, pb, 4 * n + 3), dt);
+                    if (n % A.observe_steps == 0)
+                        add_source(h, nonzero, n / A.observe_steps);
+                }
+            for (int64_t c = A.keep ? -1 : A.last > 0 ? (A.last - 1) / ck : -1; c >= 0; --c) {
                 const int64_t start = c * ck, end = std::min(start + ck, A.last);
                 HIPC(hipMemcpyAsync(A.seg.p, slot(A.cps, pb, c), pb, hipMemcpyDeviceToDevice, h->stream));
                 for (int64_t n = start + 1; n < end; ++n)
@@ -568,15 +689,7 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
                 for (int64_t n = end - 1; n >= start; --n) {
                     unsigned char* yn = slot(A.seg, pb, n - start);
                     rk4_step_into(h, yn, nullptr, dt);  // Y2, Y3, Y4 of step n
-                    // adjoint stages A2 (a), A3 (b), A4 (c) from A1 = lambda, then all four
-                    // stage pairs in one contraction, then dJ/dy_n into lambda
-                    launch_stage_buffers(h, 1, A.lam.p, A.lam.p, h->pbuf.p, A.a.p, 0.5 * dt, false, true);
-                    launch_stage_buffers(h, 2, A.a.p, A.lam.p, h->pbuf.p, A.b.p, 0.5 * dt, false, true);
-                    launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.c.p, dt, false, true);
-                    const void* mus[kPairs] = {A.lam.p, A.a.p, A.b.p, A.c.p};
-                    const void* ys[kPairs] = {A.f4.p, A.f3.p, A.f2.p, yn};
-                    launch_contract(h, mus, ys, dt);
-                    launch_stage_buffers(h, 4, A.c.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
+                    adjoint_step(h, yn, A.f2.p, A.f3.p, A.f4.p, dt);
                     if (n % A.observe_steps == 0)
                         add_source(h, nonzero, n / A.observe_steps);
                 }

@@ -79,6 +79,10 @@ struct AdjointState {
     int64_t ncp = 0;        // checkpoints, at steps 0, k, 2k, ... <= last
     std::vector<double> op; // [nel * 2] the observable O
     DevBuf<unsigned char> cps;  // [ncp] state planes
+    // whole-trajectory mode: [last][4] planes y_n, Y2, Y3, Y4 of every step n < last,
+    // kept by the forward sweep when they fit (cps then holds nothing)
+    bool keep = false;
+    DevBuf<unsigned char> traj;
 
     bool ops_ready = false;  // the L^dag constants below match the handle's operator
     bool real_h = false;

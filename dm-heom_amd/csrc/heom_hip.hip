@@ -364,9 +364,12 @@ struct TierArgs {
 // its W neighbour-table entries (folded mask) go to LDS with coalesced loads; then each
 // thread takes elements e and issues the loads of all of the element's neighbour terms
 // (the FP64 shadow for a tier-0/1 neighbour, the complex64 plane otherwise) before the
-// commutator, which reads LDS, so the gathers' latency overlaps it.
+// commutator, which reads LDS, so the gathers' latency overlaps it.  The launch is a
+// few dependent round trips per workgroup over tens of thousands of single-wavefront
+// workgroups: with two terms per element the kernel is held to 64 VGPRs (the commutator
+// loop unrolled twice, not more), so that 8 wavefronts per SIMD are resident, not 4.
 template <int STAGE, int NT>
-__global__ __launch_bounds__(256) void tier_fixup(TierArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 8 : 4))) void tier_fixup(TierArgs a)
 {
     extern __shared__ double2 tier_lds[];  // [nel] sigma, [nel] H / hbar, then [W] edges
     const int N = a.N, nel = N * N, W = a.W;
@@ -416,6 +419,7 @@ __global__ __launch_bounds__(256) void tier_fixup(TierArgs a)
             }
         }
         double2 cm = make_double2(0.0, 0.0);
+#pragma unroll 2
         for (int k = 0; k < N; ++k) {
             cm = cfma(sig[i * N + k], H[k * N + j], cm);
             const double2 hik = H[i * N + k];

@@ -344,15 +344,19 @@ int heom_set_stage_config(heom_handle* handle, const heom_tune_result* config);
  * The objective is J = sum_r sum_k Re(w_rk X_rk) with X_rk = Tr(O_k sigma_0,k(t_r)), the
  * observable O_k on the top ADO of instance k at t_r = r * observe_steps * h,
  * r = 0 .. nsteps / observe_steps.  heom_adjoint_forward propagates from the current
- * state like heom_propagate_observe, records X, keeps the state every checkpoint_steps
- * steps on the device, and leaves the state at t = nsteps * h.  heom_adjoint_backward
+ * state like heom_propagate_observe, records X and leaves the state at t = nsteps * h.
+ * It keeps the state and the three RK4 stage vectors of every step up to the last
+ * observation on the device when those 4 * L state copies (L = the last observation's
+ * step) fit in half the free device memory; otherwise, or with HEOM_ADJOINT_RECOMPUTE=1
+ * in the environment, the state every checkpoint_steps steps.  heom_adjoint_backward
  * takes the weights w (the cotangents of X) and integrates the adjoint of the RK4
- * trajectory backwards, recomputing each segment from its checkpoint, with the
- * transposed operator in hand-written kernels; it fills the gradients of J with
- * respect to the coefficient arrays of heom_desc, each complex entry as
- * dJ/dRe + i dJ/dIm.  Any pointer may be NULL to skip that output.  Setting the
- * Hamiltonian or the mask drops the record.  Device memory: (nsteps/checkpoint_steps
- * + checkpoint_steps + 7) state copies. */
+ * trajectory backwards (from checkpoints it recomputes each segment and each step's
+ * stage vectors; both modes give identical results), with the transposed operator in
+ * hand-written kernels; it fills the gradients of J with respect to the coefficient
+ * arrays of heom_desc, each complex entry as dJ/dRe + i dJ/dIm.  Any pointer may be NULL
+ * to skip that output.  Setting the Hamiltonian or the mask drops the record.  Device
+ * memory: 4 * L + 4 state copies kept, or (L / checkpoint_steps + checkpoint_steps + 7)
+ * with checkpoints. */
 typedef struct heom_gradients {
     double* hamiltonian;  /* [count][N*N*2] (per joule) */
     double* pf_same_sum;  /* [M*2] */

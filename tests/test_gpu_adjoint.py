@@ -87,9 +87,14 @@ def check_against_oracle(X, g, Xr, gr, tol=1e-10):
         assert v < tol, k
 
 
+# the forward sweep keeps every step's stage vectors when they fit on the device (keep);
+# HEOM_ADJOINT_RECOMPUTE=1 selects checkpoints and recomputation instead
+@pytest.mark.parametrize("mode", ["keep", "recompute"])
 @pytest.mark.parametrize("k", range(len(SMALL)))
 @pytest.mark.parametrize("ck", [5, 100])
-def test_gradients_match_oracle(oracle_mod, k, ck):
+def test_gradients_match_oracle(oracle_mod, monkeypatch, k, ck, mode):
+    if mode == "recompute":
+        monkeypatch.setenv("HEOM_ADJOINT_RECOMPUTE", "1")
     inst = small(SMALL[k], seed=8)
     y0, op, w, nsteps, obs = gradient_problem(inst)
     Xr, gr = inst.adjoint_gradient(y0, 1e-15, nsteps, obs, op, w, checkpoint_steps=ck)
@@ -105,6 +110,31 @@ def test_gradients_match_oracle(oracle_mod, k, ck):
     y = y0.copy()
     inst.propagate(y, 1e-15, nsteps, obs)
     assert max_abs_rel(end, y) < 1e-12
+
+
+@pytest.mark.parametrize("ck", [1, 6, 50])
+def test_trajectory_and_recompute_modes_agree_bitwise(oracle_mod, monkeypatch, ck):
+    """Both modes run the same kernels on the same inputs: the gradients, observations and
+    final states must be identical, for checkpoint intervals that do and do not divide the
+    step count and observation interval."""
+    inst = small(SMALL[1], seed=5)
+    y0, op, w, nsteps, obs = gradient_problem(inst, seed=4, nsteps=41, obs=4)
+    out = {}
+    for mode in ("keep", "recompute"):
+        if mode == "recompute":
+            monkeypatch.setenv("HEOM_ADJOINT_RECOMPUTE", "1")
+        with heom_amd.Handle.from_instance(inst) as h:
+            h.set_state(y0)
+            X = h.adjoint_forward(1e-15, nsteps, obs, op, checkpoint_steps=ck)
+            end = h.get_state()
+            g = h.adjoint_backward(w)
+            g2 = h.adjoint_backward(w[::-1].copy())
+        out[mode] = (X, end, g, g2)
+    a, b = out["keep"], out["recompute"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for ga, gb in ((a[2], b[2]), (a[3], b[3])):
+        for key in ("hamiltonian", "state") + ARRAYS:
+            assert np.array_equal(ga[key], gb[key]), key
 
 
 def test_batch_gradients_match_oracle_per_instance(oracle_mod):
