@@ -62,29 +62,12 @@ struct ContractArgs {
 };
 
 // v moved across lanes by one DPP pattern (both 32-bit halves); lanes of rows outside
-// ROWS read 0, the identity of the sum
+// ROWS read 0
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_move(double v)
 {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
-    return __hiloint2double(hi, lo);
-}
-
-// Sum over the 64 lanes, wave-uniform result.  DPP lane moves instead of LDS-crossbar
-// shuffles: pairs and quads (quad_perm), the 8-lane halves and the 16-lane rows of each
-// row (row_half_mirror, row_mirror), then rows 0 + 1 into row 1 and rows 2 + 3 into row
-// 3 (row_bcast:15), and rows 0-1 into row 3 (row_bcast:31); lane 63 holds the total.
-__device__ __forceinline__ double wave_sum(double x)
-{
-    x += dpp_move<0xb1, 0xf>(x);   // quad_perm [1, 0, 3, 2]
-    x += dpp_move<0x4e, 0xf>(x);   // quad_perm [2, 3, 0, 1]
-    x += dpp_move<0x141, 0xf>(x);  // row_half_mirror
-    x += dpp_move<0x140, 0xf>(x);  // row_mirror
-    x += dpp_move<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
-    x += dpp_move<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
-    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
     return __hiloint2double(hi, lo);
 }
 
@@ -94,15 +77,53 @@ __device__ __forceinline__ double2 cdot(double2 a, double2 b)
     return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -a.y * b.x));
 }
 
-// the wave sums of one slot, wave-uniform
-__device__ __forceinline__ double2 slot_sum(double re, double im) { return make_double2(wave_sum(re), wave_sum(im)); }
-
-// lane k of v takes the wave-uniform s
-__device__ __forceinline__ void put_lane(double2& v, int k, double2 s)
+// Sums over the 64 lanes of P values at once (P a power of two, 16 <= P <= 64).  Four
+// halving exchanges inside each 16-lane row read the lane XOR 1 and XOR 2 (quad_perm),
+// then the lane 4 and 8 places along (row_ror): the lane read holds the same set of
+// values and the opposite lane bit, each lane keeps the half its bit selects and adds
+// the other lane's copy of it, so the work halves at every exchange instead of staying at
+// P sums of six moves each.  (After the row_ror:4 exchange lane l holds x_l + x_{l+4} of
+// its half and lane l + 8 holds x_{l+8} + x_{l+12} of the same half: the row_ror:8
+// exchange adds those, each of the four lanes once.)  A lane then holds P / 16 values,
+// each summed over its row; two LDS-crossbar shuffles add the rows.  Every lane l ends
+// with the total of value bitrev4(l & 15) * P / 16 + u in x[u].
+template <int CTRL, int H>
+__device__ __forceinline__ void halve_exchange(double* x, bool upper)
 {
-    if ((int)(threadIdx.x & 63) == k)
-        v = s;
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+        const double keep = upper ? x[t + H] : x[t];
+        const double send = upper ? x[t] : x[t + H];
+        x[t] = keep + dpp_move<CTRL, 0xf>(send);
+    }
 }
+
+template <int P>
+__device__ __forceinline__ void transposed_sum(double (&x)[P])
+{
+    static_assert(P >= 16 && P <= 64 && (P & (P - 1)) == 0, "16, 32 or 64 values");
+    const int lane = threadIdx.x & 63;
+    halve_exchange<0xb1, P / 2>(x, lane & 1);           // quad_perm [1, 0, 3, 2]
+    halve_exchange<0x4e, P / 4>(x, (lane >> 1) & 1);    // quad_perm [2, 3, 0, 1]
+    halve_exchange<0x124, P / 8>(x, (lane >> 2) & 1);   // row_ror:4
+    halve_exchange<0x128, P / 16>(x, (lane >> 3) & 1);  // row_ror:8
+#pragma unroll
+    for (int u = 0; u < P / 16; ++u) {
+        x[u] += __shfl_xor(x[u], 16);
+        x[u] += __shfl_xor(x[u], 32);
+    }
+}
+
+// the value a lane holds after transposed_sum in x[u]
+template <int P>
+__device__ __forceinline__ int transposed_index(int lane, int u)
+{
+    const int t = lane & 15;
+    const int rev = ((t & 1) << 3) | ((t & 2) << 1) | ((t & 4) >> 1) | ((t & 8) >> 3);
+    return rev * (P / 16) + u;
+}
+
+constexpr int pow2_at_least_16(int v) { return v <= 16 ? 16 : v <= 32 ? 32 : 64; }
 
 // Work split as the stage kernel: workgroup = tile of 64 ADOs (lane = ADO) and one
 // stage pair (blockIdx.y), wavefront = 4 columns j of one row i; the tile of both
@@ -115,6 +136,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
     constexpr int NJW = (N + CW - 1) / CW;
     constexpr int WAVES = N * NJW;
     constexpr int SLOTS = 2 + 2 * NT;
+    constexpr int PV = pow2_at_least_16(2 * SLOTS);  // real values per element, padded
     constexpr int TB = NT < 4 ? NT : 4;  // terms whose loads are in flight together
     __shared__ double2 sy[N * N * kTile];
     __shared__ double2 sm[N * N * kTile];
@@ -146,7 +168,7 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
 
     double2 dsum = make_double2(0.0, 0.0);
     double2* out = a.part + ((int64_t)tile * kPairs + pr) * (N * N * SLOTS);
-    static_assert(SLOTS <= 64, "one lane per slot of an element");
+    static_assert(2 * SLOTS <= 64, "at most 64 real sums per element");
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
         const int j = jb + c;
@@ -154,10 +176,17 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
             break;
         const int e = i * N + j;
         const double2* plane = Y + (int64_t)e * mp;
-        // the element's SLOTS sums (consecutive in `out`), one per lane, added to `out`
-        // with one read-modify-write
-        double2 lanes = make_double2(0.0, 0.0);
-        auto put = [&](int k, double2 v) { put_lane(lanes, k, v); };
+        // the element's SLOTS complex sums (consecutive in `out`) as 2 * SLOTS real values
+        // (re, im per slot), summed over the lanes at once and added to `out` by the lanes
+        // that hold them
+        double vals[PV];
+#pragma unroll
+        for (int v = 2 * SLOTS; v < PV; ++v)
+            vals[v] = 0.0;
+        auto put = [&](int k, double re, double im) {
+            vals[2 * k] = re;
+            vals[2 * k + 1] = im;
+        };
         // neighbour loads of the first term block go out before the LDS work
         int2 nbv[TB];
         double2 xp[TB], xm[TB];
@@ -180,8 +209,8 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
         const double2 d = cdot(mm[c], ym[c]);
         dsum.x += d.x;
         dsum.y += d.y;
-        put(0, slot_sum(tt.x, tt.y));
-        put(1, slot_sum(d.x, d.y));
+        put(0, tt.x, tt.y);
+        put(1, d.x, d.y);
 #pragma unroll
         for (int k0 = 0; k0 < NT; k0 += TB) {
             if (k0 > 0) {
@@ -200,13 +229,19 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
                 const double sp = sqrt((double)(v + 1)), sn = sqrt((double)v);
                 const double2 pp = cdot(mm[c], xp[k]);  // times i sqrt(n+1)
                 const double2 m = cdot(mm[c], xm[k]);
-                put(2 + k0 + k, slot_sum(-sp * pp.y, sp * pp.x));
-                put(2 + NT + k0 + k, slot_sum(sn * m.x, sn * m.y));
+                put(2 + k0 + k, -sp * pp.y, sp * pp.x);
+                put(2 + NT + k0 + k, sn * m.x, sn * m.y);
             }
         }
-        if (lane < SLOTS) {
-            double2* o = out + e * SLOTS + lane;
-            *o = make_double2(fma(w, lanes.x, o->x), fma(w, lanes.y, o->y));
+        transposed_sum<PV>(vals);
+        if (lane < 16) {
+            double* o = reinterpret_cast<double*>(out + e * SLOTS);
+#pragma unroll
+            for (int u = 0; u < PV / 16; ++u) {
+                const int v = transposed_index<PV>(lane, u);
+                if (v < 2 * SLOTS)
+                    o[v] = fma(w, vals[u], o[v]);
+            }
         }
     }
     red[wv * kTile + lane] = dsum;
