@@ -367,9 +367,10 @@ struct TierArgs {
 // commutator, which reads LDS, so the gathers' latency overlaps it.  The launch is a
 // few dependent round trips per workgroup over tens of thousands of single-wavefront
 // workgroups: with two terms per element the kernel is held to 64 VGPRs (the commutator
-// loop unrolled twice, not more), so that 8 wavefronts per SIMD are resident, not 4.
+// loop unrolled twice, not more), so that 8 wavefronts per SIMD are resident, not 4; with
+// more terms it takes the registers it needs (no spills).
 template <int STAGE, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 8 : 4))) void tier_fixup(TierArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 8 : 1))) void tier_fixup(TierArgs a)
 {
     extern __shared__ double2 tier_lds[];  // [nel] sigma, [nel] H / hbar, then [W] edges
     const int N = a.N, nel = N * N, W = a.W;
