@@ -56,3 +56,24 @@ def test_autotune_batch_with_graphs(oracle_mod):
                 h.autotune(steps=2)
             recs.append(h.propagate(1e-15, 40, 10))
     assert max_abs_rel(recs[1], recs[0]) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pipelined_chunks_agree_bitwise(prec):
+    """The XCD chunk only changes which workgroup takes which tile and when: every chunk
+    the autotuner offers (powers of two and multiples of 24) and an odd one give the same
+    bits after five RK4 steps, on a hierarchy of 318 tiles (not a multiple of 8 chunks)."""
+    _, inst = instance("excitonic8_d8", depth=5)
+    precision = heom_amd.HEOM_PRECISION_FP32 if prec == "fp32" else heom_amd.HEOM_PRECISION_FP64
+    y = random_state(inst, 5)
+    with heom_amd.Handle.from_instance(inst, precision=precision) as h:
+        pipe = [r for r in h.autotune(steps=1) if r["kernel"] == "pipelined"]
+        assert pipe and {24, 48, 72, 96} <= {r["xcd_chunk"] for r in pipe}
+        out = {}
+        for chunk in (1, 7, 24, 32, 48, 64, 72, 96):
+            h.set_stage_config({**pipe[0], "xcd_chunk": chunk})
+            h.set_state(y)
+            h.step_rk4(1e-15, 5)
+            out[chunk] = h.get_state()
+    for chunk, v in out.items():
+        assert np.array_equal(v, out[1]), chunk
