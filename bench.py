@@ -53,6 +53,9 @@ def parse_args():
     p.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                    help="state storage of the timed run: the plain FP64 path (the headline figure) or FP32 "
                         "storage with FP64 tier-0/1 ADOs")
+    p.add_argument("--xcd-chunk", type=int, default=0,
+                   help="after the autotuner, pin this many tiles per XCD chunk in the configuration it kept "
+                        "(for profiler runs, whose timings can make the autotuner keep another chunk)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel of "
                         "known size, so that profiler byte counters can be calibrated in the same run)")
@@ -488,6 +491,10 @@ def main():
     M, N = host.matrices, host.states
     dt = info.step_size
     tuning = h.autotune(steps=3)  # the stage-kernel configuration, timed on this hierarchy
+    kept = next(r for r in tuning if r["chosen"])
+    if args.xcd_chunk > 0:
+        kept = {**kept, "xcd_chunk": args.xcd_chunk, "pinned": True}
+        h.set_stage_config(kept)
 
     h.step_rk4(dt, args.warmup)
     h.sync()
@@ -521,7 +528,6 @@ def main():
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             traffic_rec = json.load(f).get(f"{args.config}/{args.precision}")
-        kept = next(r for r in tuning if r["chosen"])
         # the traffic of a stage launch depends on the tiles per XCD chunk the tuner kept
         if traffic_rec and traffic_rec.get("xcd_chunk") in (None, kept["xcd_chunk"]):
             traffic = traffic_rec["l2_fabric_bytes_per_launch"]
@@ -570,7 +576,7 @@ def main():
         },
         "checks": {"top_trace_minus_1": trace_err},
         "pcie_inclusive": pcie,
-        "tuning": {"kept": next(r for r in tuning if r["chosen"]), "candidates": len(tuning)},
+        "tuning": {"kept": kept, "candidates": len(tuning)},
     }
     h.close()
     extras = set() if args.no_extras else {x.strip() for x in args.extras.split(",") if x.strip()}
