@@ -312,7 +312,9 @@ int heom_compact(heom_handle* handle, int64_t* active);
 
 /* The autotuner (no counterpart in the reference): times every valid configuration of
  * the stage kernels for this handle -- tile or pipelined kernel, tiles per XCD chunk of
- * the tile order and workgroups per tile of the tile kernel (N <= 8), ADOs per workgroup
+ * the tile order (powers of two up to 64, 24, 48, 72, 96 and the default: 48 for the
+ * pipelined kernel on large hierarchies, where multiples of 24 measured fastest) and
+ * workgroups per tile of the tile kernel (N <= 8), ADOs per workgroup
  * (N > 8, HEOM_STAGE_WIDE) -- over `steps`
  * RK4 steps each on the handle's own state (restored afterwards) and keeps the fastest.
  * Writes min(count, capacity) results; count receives the number of candidates. */
