@@ -137,7 +137,9 @@ __global__ __launch_bounds__(64 * stage_waves(N)) void adjoint_contract(Contract
     constexpr int WAVES = N * NJW;
     constexpr int SLOTS = 2 + 2 * NT;
     constexpr int PV = pow2_at_least_16(2 * SLOTS);  // real values per element, padded
-    constexpr int TB = NT < 4 ? NT : 4;  // terms whose loads are in flight together
+    // terms whose loads are in flight together: two once the element's sums take 32 or 64
+    // registers' worth of values (four or eight terms), so that nothing spills at N = 8
+    constexpr int TB = PV > 16 ? (NT < 2 ? NT : 2) : (NT < 4 ? NT : 4);
     __shared__ double2 sy[N * N * kTile];
     __shared__ double2 sm[N * N * kTile];
     __shared__ double2 red[WAVES * kTile];
