@@ -470,18 +470,25 @@ void rk4_step_into(heom_handle* h, const unsigned char* src, unsigned char* dst,
 // one backward RK4 step: lambda = dJ/dy_{n+1} becomes dJ/dy_n, with the step's gradient
 // contributions added to the tile sums; yn, y2, y3, y4 = the step's forward stage vectors
 void adjoint_step(heom_handle* h, const unsigned char* yn, const unsigned char* y2, const unsigned char* y3,
-                  const unsigned char* y4, double dt)
+                  const unsigned char* y4, double dt, const double2* top_src)
 {
     AdjointState& A = *h->adj;
     // adjoint stages A2 (a), A3 (b), A4 (c) from A1 = lambda, then all four stage pairs in
-    // one contraction, then dJ/dy_n into lambda
+    // one contraction, then dJ/dy_n into lambda, with the source of the observation at
+    // step n (top_src, or null) added on the top ADOs in the same launch
     launch_stage_buffers(h, 1, A.lam.p, A.lam.p, h->pbuf.p, A.a.p, 0.5 * dt, false, true);
     launch_stage_buffers(h, 2, A.a.p, A.lam.p, h->pbuf.p, A.b.p, 0.5 * dt, false, true);
     launch_stage_buffers(h, 3, A.b.p, A.lam.p, h->pbuf.p, A.c.p, dt, false, true);
     const void* mus[kPairs] = {A.lam.p, A.a.p, A.b.p, A.c.p};
     const void* ys[kPairs] = {y4, y3, y2, yn};
     launch_contract(h, mus, ys, dt);
-    launch_stage_buffers(h, 4, A.c.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true);
+    launch_stage_buffers(h, 4, A.c.p, A.lam.p, h->pbuf.p, A.lam.p, dt / 6.0, false, true, top_src);
+}
+
+// the adjoint source of observation row `row` (null for a row whose weights are all zero)
+const double2* source_of(heom_handle* h, const std::vector<char>& nonzero, int64_t row)
+{
+    return nonzero[row] ? h->adj->src.p + (size_t)row * h->count * h->nel : nullptr;
 }
 
 // observation row `row` of the forward sweep from the state planes y
@@ -712,12 +719,10 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
             HIPC(hipEventRecord(e0, h->stream));
             add_source(h, nonzero, A.rows - 1);
             if (A.keep)
-                for (int64_t n = A.last - 1; n >= 0; --n) {
+                for (int64_t n = A.last - 1; n >= 0; --n)
                     adjoint_step(h, slot(A.traj, pb, 4 * n), slot(A.traj, pb, 4 * n + 1), slot(A.traj, pb, 4 * n + 2),
-                                 slot(A.traj, pb, 4 * n + 3), dt);
-                    if (n % A.observe_steps == 0)
-                        add_source(h, nonzero, n / A.observe_steps);
-                }
+                                 slot(A.traj, pb, 4 * n + 3), dt,
+                                 n % A.observe_steps == 0 ? source_of(h, nonzero, n / A.observe_steps) : nullptr);
             for (int64_t c = A.keep ? -1 : A.last > 0 ? (A.last - 1) / ck : -1; c >= 0; --c) {
                 const int64_t start = c * ck, end = std::min(start + ck, A.last);
                 HIPC(hipMemcpyAsync(A.seg.p, slot(A.cps, pb, c), pb, hipMemcpyDeviceToDevice, h->stream));
@@ -726,9 +731,8 @@ int heom_adjoint_backward(heom_handle* h, const double* weights, heom_gradients*
                 for (int64_t n = end - 1; n >= start; --n) {
                     unsigned char* yn = slot(A.seg, pb, n - start);
                     rk4_step_into(h, yn, nullptr, dt);  // Y2, Y3, Y4 of step n
-                    adjoint_step(h, yn, A.f2.p, A.f3.p, A.f4.p, dt);
-                    if (n % A.observe_steps == 0)
-                        add_source(h, nonzero, n / A.observe_steps);
+                    adjoint_step(h, yn, A.f2.p, A.f3.p, A.f4.p, dt,
+                                 n % A.observe_steps == 0 ? source_of(h, nonzero, n / A.observe_steps) : nullptr);
                 }
             }
             HIPC(hipEventRecord(e1, h->stream));

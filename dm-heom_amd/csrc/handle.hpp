@@ -243,9 +243,10 @@ inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + thre
 
 // one stage launch of the FP64 RK4 schedule with explicit buffers: yin the stage input
 // (neighbours are read there), y0 = y_n (stages 2, 3), pbuf = P (written by stage 2, read
-// by stage 4), yout the stage output; adjoint = true applies L^dag instead of L
+// by stage 4), yout the stage output; adjoint = true applies L^dag instead of L, and then
+// top_src ([count][N*N], stage 4 only) is added to the output at each instance's top ADO
 void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void* y0, void* pbuf, void* yout,
-                          double c1, bool reverse, bool adjoint);
+                          double c1, bool reverse, bool adjoint, const double2* top_src = nullptr);
 // builds the device layout of count instances into a fresh handle (heom_create_batch)
 void create_batch(heom_handle* h, const heom_desc* d, int count);
 // raises HEOM_E_UNSUPPORTED for `what` on a thermal-state-search handle

@@ -106,6 +106,10 @@ struct StageArgsT {
     const int32_t* __restrict__ tile_inst;  // [ntiles] instance of each tile, or null (one instance)
     const int32_t* __restrict__ tile_plus;  // [ntiles] 0 if no ADO of the tile has a plus edge, or null
     const int32_t* __restrict__ tile_class; // [ntiles] bath class of each tile (element constants), or null
+    // adjoint, fourth stage: [count][N*N] added to the output at each instance's top ADO
+    // (slot top_q[k]), or null
+    const c2<R>* __restrict__ top_src;
+    const int64_t* __restrict__ top_q;
     int64_t mp;
     int32_t ntiles;       // Mp / 64
     int32_t zero_slot;    // index of the zero slot (M)

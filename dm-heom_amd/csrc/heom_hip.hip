@@ -540,13 +540,17 @@ void launch_stage_r(heom_handle* h, int stage, const unsigned char* yin, unsigne
 }
 
 void launch_stage_buffers(heom_handle* h, int stage, const void* yin, const void* y0, void* pbuf, void* yout,
-                          double c1, bool reverse, bool adjoint)
+                          double c1, bool reverse, bool adjoint, const double2* top_src)
 {
     if (h->fp32())
         raise(HEOM_E_UNSUPPORTED, "explicit-buffer stages need FP64 storage");
+    if (top_src && !(adjoint && stage == 4))
+        raise(HEOM_E_INVALID, "a top source goes with the adjoint's fourth stage only");
     if (adjoint) {
         AdjointState& A = *h->adj;
-        const StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, A.pfss.p);
+        StageArgsT<double> a = stage_args<double>(h, yin, y0, pbuf, yout, c1, reverse, A.pfss.p);
+        a.top_src = top_src;
+        a.top_q = h->q_top.p;
         const ElementArgsT<double> el{A.hs.p, A.gam.p, A.terms.p};
         heom_hip::launch_stage_adjoint(h->N, h->term_slots, A.real_h, stage, a, el, h->stream);
     } else {

@@ -61,6 +61,12 @@ __device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* 
     const C* hs = tile_hamiltonian<R, N>(hs_all, a.tile_inst, tile);
     const int64_t mp = a.mp;
     const uint32_t q = (uint32_t)tile * kTile + lane;
+    // the adjoint's fourth stage: the top ADO slot of the tile's instance, read up front
+    // (a wave-uniform load that the epilogue would otherwise wait for), -1 without a source
+    int64_t qtop = -1;
+    if constexpr (ADJ && STAGE == 4)
+        if (a.top_src)
+            qtop = a.top_q[a.tile_inst ? a.tile_inst[tile] : 0];
 
     C s[CW];
 #pragma unroll
@@ -174,7 +180,20 @@ __device__ __forceinline__ void stage_tile(const StageArgsT<R>& a, const c2<R>* 
             }
         }
 
-        if (kHoist) {
+        if constexpr (ADJ && STAGE == 4) {
+            // the observation source on the top ADO, added to the stored value as a launch
+            // of its own would add it: the same bits, one launch less per step
+            C r0, r1 = mk2<R>(R(0), R(0));
+            stage_values<STAGE, R>(a.c1, s[c], f,
+                                   kHoist ? eph[kHoist ? c : 0] : stage_operand<STAGE, R>(a, (int64_t)e * mp + q), r0,
+                                   r1);
+            if ((int64_t)q == qtop) {
+                const C src = a.top_src[(int64_t)(a.tile_inst ? a.tile_inst[tile] : 0) * (N * N) + e];
+                r0.x += src.x;
+                r0.y += src.y;
+            }
+            stage_store<STAGE, R>(a, (int64_t)e * mp + q, r0, r1);
+        } else if (kHoist) {
             C r0, r1 = mk2<R>(R(0), R(0));
             stage_values<STAGE, R>(a.c1, s[c], f, eph[kHoist ? c : 0], r0, r1);
             stage_store<STAGE, R>(a, (int64_t)e * mp + q, r0, r1);
