@@ -333,8 +333,8 @@ int heom_autotune(heom_handle* handle, int32_t steps, heom_tune_result* results,
  * counterpart in the reference): a result of heom_autotune on a handle of the same
  * shape, so a tuned configuration can be reused.  kernel, workgroups (pipelined kernel)
  * and ados_per_workgroup must be a candidate heom_autotune would time for this handle;
- * xcd_chunk lies in [1, tiles]; chosen and ms_per_step are ignored.  Results do not
- * depend on the configuration. */
+ * xcd_chunk lies in [1, tiles]; chosen and ms_per_step are ignored.  The tiles per XCD
+ * chunk do not change a bit of the results; the kernels agree to rounding. */
 int heom_set_stage_config(heom_handle* handle, const heom_tune_result* config);
 
 /* ---- adjoint: gradients of observables ------------------------------------------ */
