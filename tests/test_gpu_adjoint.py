@@ -33,6 +33,8 @@ SMALL = [
     dict(N=3, K=2, D=2, coupling=[[0], [1], [2]], omega=[0, 120, -80], complex_h=True),
     dict(N=3, K=1, D=2, coupling=[[-1], [0], [1]]),
     dict(N=3, K=1, D=2, coupling=[[0, 1], [1, -1], [0, -1]]),
+    # eight sites with two Matsubara terms per bath: four terms per element at N = 8
+    dict(N=8, K=2, D=2, coupling=[[i] for i in range(8)]),
 ]
 
 
