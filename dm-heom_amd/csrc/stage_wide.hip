@@ -157,7 +157,17 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
             for (int c = 0; c < B; ++c) {
                 const int i = ir[r], j = jc[c];
                 const C g = gam[i * N + j];
-                fv[r][c] = cfma(mk2<R>(S.x + g.x, S.y + g.y), s[i * NP + j], mk2<R>(-cm[r][c].y, cm[r][c].x));
+                const C own = s[i * NP + j];
+                const C fb = cfma(mk2<R>(S.x + g.x, S.y + g.y), own, mk2<R>(-cm[r][c].y, cm[r][c].x));
+                // stages 1 and 4 take the element's own value linearly (s + c1 f, and
+                // (ep + s) / 3 + c1 f): its part goes into the LDS value here, so that
+                // phase C needs no second read of it
+                if (STAGE == 1)
+                    fv[r][c] = mk2<R>(fma(a.c1, fb.x, own.x), fma(a.c1, fb.y, own.y));
+                else if (STAGE == 4)
+                    fv[r][c] = mk2<R>(fma(own.x, R(1) / R(3), a.c1 * fb.x), fma(own.y, R(1) / R(3), a.c1 * fb.y));
+                else
+                    fv[r][c] = fb;
             }
         __syncthreads();  // every wavefront is done reading the tile
         C* fo = sig + wv * mat;
@@ -196,7 +206,8 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
                 for (int k = 0; k < NT; ++k)
                     nbv[u][k] = tabl[terms[e * NT + k].mode * T + ad];
                 ep[u] = stage_operand<STAGE, R>(a, o[u]);
-                own[u] = a.yin[o[u]];
+                if (STAGE == 2)  // stages 1 and 4 have it in the LDS value, 0 and 3 do not use it
+                    own[u] = a.yin[o[u]];
             }
 #pragma unroll
             for (int u = 0; u < PP; ++u) {
@@ -212,7 +223,10 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
             for (int u = 0; u < PP; ++u) {
                 if (!live[u])
                     continue;
-                C f = sig[lo[u]];
+                // stages 1 and 4: the LDS value already holds the own part and c1 times the
+                // commutator and same terms; f gathers the neighbour terms alone
+                const C lv = sig[lo[u]];
+                C f = (STAGE == 1 || STAGE == 4) ? mk2<R>(R(0), R(0)) : lv;
 #pragma unroll
                 for (int k = 0; k < NT; ++k) {
                     const TermT<R> t = terms[e2[u] * NT + k];
@@ -226,7 +240,14 @@ __global__ __launch_bounds__(kWideThreads) void rk4_stage_wide(StageArgsT<R> a, 
                     f = cfma(t.minus_c, mk2<R>(sm * xm[u][k].x, sm * xm[u][k].y), f);
                 }
                 C r0, r1 = mk2<R>(R(0), R(0));
-                stage_values<STAGE, R>(a.c1, own[u], f, ep[u], r0, r1);
+                if (STAGE == 1) {
+                    r0 = mk2<R>(fma(a.c1, f.x, lv.x), fma(a.c1, f.y, lv.y));
+                } else if (STAGE == 4) {
+                    constexpr R third = R(1) / R(3);
+                    r0 = mk2<R>(fma(ep[u].x, third, fma(a.c1, f.x, lv.x)), fma(ep[u].y, third, fma(a.c1, f.y, lv.y)));
+                } else {
+                    stage_values<STAGE, R>(a.c1, STAGE == 2 ? own[u] : mk2<R>(R(0), R(0)), f, ep[u], r0, r1);
+                }
                 stage_store<STAGE, R>(a, o[u], r0, r1);
             }
         }
