@@ -528,9 +528,11 @@ def main():
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             traffic_rec = json.load(f).get(f"{args.config}/{args.precision}")
-        # the traffic of a stage launch depends on the tiles per XCD chunk the tuner kept
-        if traffic_rec and traffic_rec.get("xcd_chunk") in (None, kept["xcd_chunk"]):
-            traffic = traffic_rec["l2_fabric_bytes_per_launch"]
+        # the traffic of a stage launch depends on the tiles per XCD chunk the tuner kept:
+        # the record holds the chunks the tuner keeps on this hierarchy
+        chunk_rec = (traffic_rec or {}).get("by_chunk", {}).get(str(kept["xcd_chunk"]))
+        if chunk_rec:
+            traffic = chunk_rec["l2_fabric_bytes_per_launch"]
 
     out = {
         "metric": "ADO\u00b7RK4-steps/s + % HBM peak; batched FP32 vs plain FP64 at \u22641e-6 pop err",
@@ -567,7 +569,7 @@ def main():
                 "request is 128 B on gfx950, so TCC_EA0_RDREQ x 128 B and FETCH_SIZE x 2 KiB count the same bytes); it "
                 "includes hits in the 256 MB last-level cache (MALL), which no L2-side counter separates from "
                 f"DRAM (TCC_EA0_RDREQ_DRAM equals TCC_EA0_RDREQ); measured on build {traffic_rec.get('build')} with "
-                f"{traffic_rec.get('xcd_chunk')} tiles per XCD chunk, {traffic_rec.get('source')}"),
+                f"{kept['xcd_chunk']} tiles per XCD chunk, {traffic_rec.get('source')}"),
             "kernel": f"{stage_kernel_name(h, fp32)} (RHS + RK4 stage epilogue), 4 launches per step",
             "alg_bytes_per_launch": alg_bytes_per_launch,
             "avg_launch_ms": per_launch_s * 1e3,
